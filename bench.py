@@ -1,0 +1,257 @@
+"""Benchmark of the two-site DMRG hot path on one MI355X.
+
+Workload (J1-J2 chain, MPO bond dimension w = 8, physical dim d = 2, bond
+dimension D = 1024 - the precision configuration of the reference's
+benchmark set): one "step" is one two-site H_eff matvec
+(apply_two_site_heff, reference src/algorithms/dmrg.mojo:254-374) on
+HBM-resident L, R, W and theta.  The matvec is timed in FP64 (the
+oracle-equivalent path), FP32 and 3xBF16; ``value`` is the 3xBF16 matvec rate
+in algorithmic TFLOP/s (reference contraction order, 139.59 GFLOP per matvec).
+
+Also reported: a fixed-D DMRG sweep (FP64 and 3xBF16 + FP64 refinement) with
+its energy difference, the roofline of the dominant kernel (the last GEMM of
+the matvec, U[(Dl d d)][(wR Dr)] R[(wR Dr)][Dr']) timed live with events on
+the stream it runs on, and the FP64 CPU oracle timed on the host cores.
+
+Prints one JSON line.  N > 1 GPUs is not supported (the path runs on one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DMRG sweep s + H_eff matvec TFLOP/s at D=1024; |ΔE/E| vs FP64 oracle"
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # TFLOP/s, MI355X dense MFMA
+HBM_PEAK = 8000.0  # GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def matvec_flops(Dl, d, Dr, wL, wM, wR):
+    return (2.0 * wL * Dl * Dl * d * d * Dr + 2.0 * Dl * d * Dr * wL * d * d * wM
+            + 2.0 * Dl * Dr * d * wM * d * d * wR + 2.0 * Dl * d * d * wR * Dr * Dr)
+
+
+def make_problem(D, w, seed=42):
+    from chemtensor_mojo_amd import hamiltonians as ham
+    rng = np.random.default_rng(seed)
+    L = rng.standard_normal((w, D, D)) / np.sqrt(D)
+    L = 0.5 * (L + L.transpose(0, 2, 1))
+    R = rng.standard_normal((w, D, D)) / np.sqrt(D)
+    R = 0.5 * (R + R.transpose(0, 2, 1))
+    W = ham.j1j2_tensors(6, 1.0, 0.5)[2]  # bulk tensor [8, 2, 2, 8]
+    th = rng.standard_normal((D, 2, 2, D))
+    th /= np.linalg.norm(th)
+    return L, R, W, th
+
+
+def time_events(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / 1e3 / steps, wall
+
+
+def bench_matvec(ctx, D, w, steps, warmup):
+    import torch
+    import chemtensor_mojo_amd as ct
+    L, R, W, th = make_problem(D, w)
+    dev = lambda a: ct.create_dense_tensor_from_data(ctx, a, a.shape)
+    tL, tR, tW, tth = dev(L), dev(R), ct.MPOSite(dev(W)), dev(th)
+    F = matvec_flops(D, 2, D, w, w, w)
+    res = {}
+    outs = {}
+    for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3)):
+        out_holder = {}
+
+        def step():
+            out_holder["o"] = ct.apply_two_site_heff(tth, tL, tR, tW, tW, ctx, prec=prec)
+        t, wall = time_events(step, steps, warmup)
+        outs[name] = out_holder["o"].to_numpy()
+        res[name] = {"s_per_matvec": t, "tflops": F / t / 1e12}
+        log(f"matvec {name}: {t * 1e3:.3f} ms  {F / t / 1e12:.2f} TFLOP/s")
+    ref = outs["fp64"]
+    for name in ("fp32", "bf16x3"):
+        res[name]["rel_err_vs_fp64"] = float(np.linalg.norm(outs[name] - ref) / np.linalg.norm(ref))
+    del tL, tR, tth
+    torch.cuda.empty_cache()
+    return res, F
+
+
+def bench_dominant_gemm(ctx, D, w, steps, warmup):
+    """Stage-4 GEMM of the matvec alone: [4D x wD] @ [wD x D], the same kernels."""
+    import torch
+    import chemtensor_mojo_amd as ct
+    from chemtensor_mojo_amd import _lib
+    M, K, N = 4 * D, w * D, D
+    F = 2.0 * M * N * K
+    g = torch.Generator(device="cuda").manual_seed(1)
+    A64 = torch.randn(M * K, dtype=torch.float64, device="cuda", generator=g)
+    B64 = torch.randn(K * N, dtype=torch.float64, device="cuda", generator=g)
+    out = {}
+    # FP64: ct_tensordot of A[M,K] . B[K,N] (A k-contiguous, B n-contiguous: the heff stage-4 kernel)
+    tA = ct.DenseTensor(A64, [M, K])
+    tB = ct.DenseTensor(B64, [K, N])
+    tC = ct.create_dense_tensor(ctx, [M, N])
+    t, _ = time_events(lambda: ct.dense_tensor_dot(tC, tA, tB, ctx), steps, warmup)
+    out["fp64"] = {"s": t, "tflops": F / t / 1e12, "bytes": 8.0 * (M * K + K * N + M * N)}
+    # 3xBF16: pre-split planes, time only the NT MFMA kernel
+    Ah = torch.empty(M * K, dtype=torch.int16, device="cuda")
+    Al = torch.empty_like(Ah)
+    Bh = torch.empty(N * K, dtype=torch.int16, device="cuda")
+    Bl = torch.empty_like(Bh)
+    vp = lambda x: ctypes.c_void_p(x.data_ptr())
+    _lib.call("ct_split_bf16", ctx.handle, 0, vp(A64), M, K, K, 1, vp(Ah), vp(Al), K)
+    _lib.call("ct_split_bf16", ctx.handle, 0, vp(B64), N, K, 1, N, vp(Bh), vp(Bl), K)
+    C32 = torch.empty(M * N, dtype=torch.float64, device="cuda")
+    t, _ = time_events(lambda: _lib.call("ct_gemm_bf16x3_nt", ctx.handle, vp(Ah), vp(Al), K, vp(Bh), vp(Bl), K, 0,
+                                         vp(C32), M, N, K, N), steps, warmup)
+    out["bf16x3"] = {"s": t, "tflops": F / t / 1e12, "mfma_tflops": 3 * F / t / 1e12,
+                     "bytes": 4.0 * (M * K + K * N) + 8.0 * M * N}
+    ref = (A64.view(M, K) @ B64.view(K, N)).reshape(-1)
+    out["bf16x3"]["rel_err"] = float(torch.linalg.norm(C32 - ref) / torch.linalg.norm(ref))
+    out["fp64"]["rel_err"] = float(torch.linalg.norm(tC.storage - ref) / torch.linalg.norm(ref))
+    for k, v in out.items():
+        log(f"dominant GEMM {k}: {v['s'] * 1e3:.3f} ms {v['tflops']:.2f} TFLOP/s (err {v['rel_err']:.1e})")
+    del A64, B64, Ah, Al, Bh, Bl, C32, tC
+    torch.cuda.empty_cache()
+    return out, F
+
+
+def bench_sweep(ctx, L_sites, D, prec, refine, seed=42, sweeps=2, jacobi_max_rot=20000):
+    import chemtensor_mojo_amd as ct
+    from chemtensor_mojo_amd import hamiltonians as ham
+    mpo = ct.mpo_from_tensors(ctx, ham.j1j2_tensors(L_sites, 1.0, 0.5))
+    mps = ct.create_random_mps_c_compatible(ctx, L_sites, 2, 32, seed)
+    p = ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=0.0, max_krylov_iter=20, krylov_tol=1e-12,
+                      energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=jacobi_max_rot)
+    run = ct.DMRGRun(ctx, mpo, mps, p)
+    try:
+        Es = []
+        times = []
+        for s in range(sweeps):
+            t0 = time.perf_counter()
+            E, _ = run.run(1)
+            times.append(time.perf_counter() - t0)
+            Es.append(E)
+            st = run.stats()
+            log(f"sweep {s} prec={prec} refine={refine}: {times[-1]:.2f} s E={E:.12f} maxD={st.max_bond}")
+        st = run.stats()
+    finally:
+        run.close()
+    return {"energies": Es, "sweep_s": times, "t_lanczos": st.t_lanczos, "t_svd": st.t_svd, "t_env": st.t_env,
+            "t_theta": st.t_theta, "matvecs": st.matvecs, "matvec_flops": st.matvec_flops, "max_bond": st.max_bond,
+            "trunc_weight_sum": st.trunc_weight_sum}
+
+
+def cpu_baseline(D, w, budget_s):
+    from oracle import oracle as ora
+    ora.build()
+    n = os.cpu_count() or 1
+    ora.set_num_threads(n)
+    L, R, W, th = make_problem(D, w)
+    F = matvec_flops(D, 2, D, w, w, w)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        ora.heff_apply(L, W, W, R, th)
+        reps += 1
+        if time.perf_counter() - t0 > budget_s or reps >= 20:
+            break
+    t = (time.perf_counter() - t0) / reps
+    return {"value": F / t / 1e12, "unit": "TFLOP/s", "cores": ora.get_num_threads(), "kind": "port",
+            "sample": f"{reps} FP64 H_eff matvecs at D={D}, w={w} (oracle, OpenBLAS dgemm, {t:.2f} s each)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--D", type=int, default=1024)
+    ap.add_argument("--w", type=int, default=8)
+    ap.add_argument("--sweep-sites", type=int, default=16)
+    ap.add_argument("--sweep-D", type=int, default=256)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this path runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design note only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import chemtensor_mojo_amd as ct
+    ctx = ct.DeviceContext(0)
+    D, w = args.D, args.w
+    log(f"device: {torch.cuda.get_device_name(0)}")
+    mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)
+    gm, Fg = bench_dominant_gemm(ctx, D, w, args.steps, args.warmup)
+    sweep = None
+    if not args.no_sweep:
+        s64 = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_FP64, 0)
+        smx = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_BF16X3, 2)
+        E64, Emx = s64["energies"][-1], smx["energies"][-1]
+        sweep = {"L": args.sweep_sites, "D": args.sweep_D, "w": 8, "fp64": s64, "bf16x3_refined": smx,
+                 "dE_rel": abs(Emx - E64) / abs(E64),
+                 "speedup_sweep": s64["sweep_s"][-1] / smx["sweep_s"][-1]}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(D, w, args.cpu_budget)
+        log(f"cpu baseline: {cpu['value']:.3f} TFLOP/s on {cpu['cores']} threads")
+    head = mv["bf16x3"]
+    dom = gm["bf16x3"]
+    line = {
+        "metric": METRIC,
+        "value": round(head["tflops"], 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["s_per_matvec"] * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3 (fp32 accumulate)",
+        "data": "synthetic (seeded random environments/theta, J1-J2 bulk MPO)",
+        "config": {"workload": f"two-site H_eff matvec, J1-J2 chain, D={D}, w={w}, d=2",
+                   "matvec_gflop": round(F / 1e9, 3)},
+        "matvec": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                   for k, v in mv.items()},
+        "roofline": {"bound": "mfma", "kernel": "gemm_bf16x3_nt (stage-4 GEMM of the matvec)",
+                     "achieved": round(dom["tflops"], 3), "peak": PEAK["bf16"] / 3.0, "unit": "TFLOP/s",
+                     "frac": round(dom["tflops"] / (PEAK["bf16"] / 3.0), 4), "traffic": None,
+                     "note": "3 BF16 MFMA products per algorithmic FLOP: peak = 2500/3 TFLOP/s"},
+        "roofline_fp64": {"bound": "mfma", "achieved": round(gm["fp64"]["tflops"], 3), "peak": PEAK["fp64"],
+                          "unit": "TFLOP/s", "frac": round(gm["fp64"]["tflops"] / PEAK["fp64"], 4)},
+        "sweep": sweep,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,20 @@
+"""chemtensor_mojo_amd: the two-site DMRG hot path of chemtensor_mojo on MI355X (gfx950).
+
+Everything numeric runs in libdmrg_hip.so (hand-written HIP kernels for CDNA4:
+MFMA GEMMs in FP64 / FP32 / 3xBF16, MPO mixing, device Lanczos, rocSOLVER
+SVD, C++ sweep driver).  The Python modules mirror the reference's Mojo call
+surface (src/m_tensor, src/state, src/algorithms) on top of the C ABI.
+"""
+from ._lib import PREC_BF16X3, PREC_FP32, PREC_FP64
+from .dense_tensor import (DenseTensor, DeviceContext, axpy_in_place, compute_row_major_strides, create_dense_tensor,
+                           create_dense_tensor_from_data, create_dense_tensor_uninitialized, dense_tensor_dot,
+                           dot_product)
+from .dmrg import (DMRGParams, DMRGRun, DMRGStats, apply_two_site_heff, build_two_site_theta, dense_tensor_svd_trunc,
+                   dmrg_two_site, lanczos_two_site_optimize, split_two_site_theta_svd)
+from .state import (MatrixProductOperator, MatrixProductState, MPOSite, MPSSite, build_right_environments,
+                    create_product_mps, create_random_mps_c_compatible, expectation_value, mpo_from_tensors,
+                    mps_from_tensors, random_mps_c_compatible_tensors, update_left_environment,
+                    update_right_environment)
+from . import hamiltonians
+
+__version__ = "0.1.0"

@@ -1,0 +1,277 @@
+// extern "C" entry points of libdmrg_hip.so (declared in include/chemtensor_hip.h)
+// and the handle / workspace management behind them.
+#include "ct_internal.h"
+
+namespace ct {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+int ws_get(Handle* h, int slot, size_t bytes, void** out) {
+    Buffer& b = h->ws[slot];
+    if (bytes == 0) bytes = 16;
+    if (b.bytes < bytes) {
+        // earlier work on the stream may still read the old buffer
+        CT_HIP(hipStreamSynchronize(h->stream));
+        if (b.ptr) CT_HIP(hipFree(b.ptr));
+        b.ptr = nullptr;
+        b.bytes = 0;
+        size_t want = bytes + bytes / 8;  // slack for slowly growing bond dimensions
+        want = (want + 4095) & ~(size_t)4095;
+        hipError_t e = hipMalloc(&b.ptr, want);
+        if (e != hipSuccess) {
+            b.ptr = nullptr;
+            set_last_error("workspace allocation of " + std::to_string(want) + " bytes failed: " +
+                           hipGetErrorString(e));
+            return CT_STATUS_ALLOC;
+        }
+        b.bytes = want;
+    }
+    *out = b.ptr;
+    return 0;
+}
+
+}  // namespace ct
+
+static constexpr size_t kPinnedBytes = 1 << 20;
+
+extern "C" {
+
+const char* ct_last_error(void) { return ct::g_last_error.c_str(); }
+
+const char* ct_version(void) { return "chemtensor-mi355x 0.1 (gfx950)"; }
+
+int ct_handle_create(int device, void* stream, ct_handle_t* out) {
+    CT_ARG(out != nullptr, 3, "out is NULL");
+    int ndev = 0;
+    CT_HIP(hipGetDeviceCount(&ndev));
+    CT_ARG(device >= 0 && device < ndev, 1, "no such device");
+    CT_HIP(hipSetDevice(device));
+    ct_handle_s* h = new ct_handle_s();
+    h->device = device;
+    if (stream) {
+        h->stream = (hipStream_t)stream;
+        h->own_stream = false;
+    } else {
+        CT_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->own_stream = true;
+    }
+    if (rocblas_create_handle(&h->blas) != rocblas_status_success) {
+        delete h;
+        ct::set_last_error("rocblas_create_handle failed");
+        return CT_STATUS_ROCBLAS_ERROR_BASE;
+    }
+    rocblas_set_stream(h->blas, h->stream);
+    CT_HIP(hipMalloc(&h->d_scalars, 4096));
+    CT_HIP(hipHostMalloc(&h->h_pinned, kPinnedBytes, hipHostMallocDefault));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
+    *out = h;
+    return 0;
+}
+
+int ct_handle_destroy(ct_handle_t h) {
+    if (!h) return 0;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    for (auto& b : h->ws)
+        if (b.ptr) (void)hipFree(b.ptr);
+    if (h->d_scalars) (void)hipFree(h->d_scalars);
+    if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+    if (h->blas) rocblas_destroy_handle(h->blas);
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int ct_handle_set_stream(ct_handle_t h, void* stream) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_HIP(hipStreamSynchronize(h->stream));
+    if (h->own_stream) CT_HIP(hipStreamDestroy(h->stream));
+    if (stream) {
+        h->stream = (hipStream_t)stream;
+        h->own_stream = false;
+    } else {
+        CT_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->own_stream = true;
+    }
+    rocblas_set_stream(h->blas, h->stream);
+    return 0;
+}
+
+int ct_synchronize(ct_handle_t h) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int ct_tensordot(ct_handle_t h, int prec, int dtype, const void* A, int rankA, const int64_t* shapeA, const void* B,
+                 int rankB, const int64_t* shapeB, int ndim_mult, int axrange_A, int axrange_B, void* C, int rankC,
+                 const int64_t* shapeC) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
+    return ct::tensordot(h, prec, dtype, A, rankA, shapeA, B, rankB, shapeB, ndim_mult, axrange_A, axrange_B, C,
+                         rankC, shapeC);
+}
+
+int ct_split_bf16(ct_handle_t h, int dtype, const void* X, int64_t rows, int64_t K, int64_t s_row, int64_t s_k,
+                  uint16_t* hi, uint16_t* lo, int64_t ld) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 2, "unknown dtype");
+    CT_ARG(rows >= 0 && rows < (1ll << 31), 4, "rows out of range");
+    CT_ARG(K >= 0 && K < (1ll << 31), 5, "K out of range");
+    CT_ARG(ld >= K, 10, "ld must be >= K");
+    return ct::split_bf16(h, dtype, X, (int)rows, (int)K, s_row, s_k, 1, 0, hi, lo, ld, 0, nullptr);
+}
+
+int ct_gemm_bf16x3_nt(ct_handle_t h, const uint16_t* A_hi, const uint16_t* A_lo, int64_t lda, const uint16_t* B_hi,
+                      const uint16_t* B_lo, int64_t ldb, int dtype_c, void* C, int64_t M, int64_t N, int64_t K,
+                      int64_t ldc) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(lda >= K, 4, "lda must be >= K");
+    CT_ARG(ldb >= K, 7, "ldb must be >= K");
+    CT_ARG(dtype_c == CT_F64 || dtype_c == CT_F32, 8, "unknown dtype");
+    CT_ARG(M >= 0 && M < (1ll << 31), 10, "M out of range");
+    CT_ARG(N >= 0 && N < (1ll << 31), 11, "N out of range");
+    CT_ARG(K >= 0 && K < (1ll << 31), 12, "K out of range");
+    CT_ARG(ldc >= N, 13, "ldc must be >= N");
+    ct::SplitOp a, b;
+    a.hi = A_hi; a.lo = A_lo; a.ld = lda;
+    b.hi = B_hi; b.lo = B_lo; b.ld = ldb;
+    return ct::gemm_bf16x3_nt(h, a, b, dtype_c, C, (int)M, (int)N, (int)K, ldc, 0, 1, 1.0, 0.0, nullptr);
+}
+
+int ct_permute(ct_handle_t h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm, void* out) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 2, "unknown dtype");
+    CT_ARG(rank >= 1 && rank <= 8, 4, "rank must be 1..8");
+    bool seen[8] = {false};
+    for (int i = 0; i < rank; ++i) {
+        CT_ARG(perm[i] >= 0 && perm[i] < rank && !seen[perm[i]], 6, "perm is not a permutation");
+        seen[perm[i]] = true;
+    }
+    return ct::permute(h, dtype, in, rank, shape, perm, out);
+}
+
+int ct_norm_sq(ct_handle_t h, int dtype, const void* x, int64_t n, double* result) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(result != nullptr, 5, "result is NULL");
+    return ct::dot_host(h, dtype, x, x, n, result);
+}
+
+int ct_dot(ct_handle_t h, int dtype, const void* x, const void* y, int64_t n, double* result) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(result != nullptr, 6, "result is NULL");
+    return ct::dot_host(h, dtype, x, y, n, result);
+}
+
+int ct_axpy(ct_handle_t h, int dtype, double alpha, const void* x, void* y, int64_t n) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    return ct::axpy(h, dtype, alpha, x, y, n);
+}
+
+int ct_scale(ct_handle_t h, int dtype, double alpha, void* x, int64_t n) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    return ct::scale(h, dtype, alpha, x, n);
+}
+
+int ct_convert(ct_handle_t h, int dtype_in, const void* in, int dtype_out, void* out, int64_t n) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    return ct::convert(h, dtype_in, in, dtype_out, out, n);
+}
+
+int ct_heff_apply(ct_handle_t h, int prec, int dtype, const void* L, const void* Wi, const void* Wj, const void* R,
+                  const void* theta, void* out, int64_t Dl, int64_t Dlb, int64_t d, int64_t Dr, int64_t Drb,
+                  int64_t wL, int64_t wM, int64_t wR) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
+    CT_ARG(Dl > 0 && Dlb > 0 && d > 0 && Dr > 0 && Drb > 0 && wL > 0 && wM > 0 && wR > 0, 11, "dimensions must be > 0");
+    ct::HeffDims hd{Dl, Dlb, d, Dr, Drb, wL, wM, wR};
+    void* w2;
+    CT_CHECK(ct::ws_get(h, ct::WS_W2, (size_t)(wL * d * d * d * d * wR) * 8, &w2));
+    CT_CHECK(ct::heff_build_w2(h, dtype, Wi, Wj, wL, wM, wR, d, (double*)w2));
+    ct::HeffEnv env;
+    CT_CHECK(ct::heff_prepare(h, prec, dtype, L, R, (const double*)w2, hd, &env));
+    return ct::heff_apply(h, env, dtype, theta, out, nullptr);
+}
+
+int ct_env_update_left(ct_handle_t h, int prec, int dtype, const void* L, const void* A, const void* W, void* out,
+                       int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
+    CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wR > 0, 8, "dimensions must be > 0");
+    return ct::env_left(h, prec, dtype, L, A, W, out, Dl, d, Dr, wL, wR);
+}
+
+int ct_env_update_right(ct_handle_t h, int prec, int dtype, const void* R, const void* A, const void* W, void* out,
+                        int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
+    CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wR > 0, 8, "dimensions must be > 0");
+    return ct::env_right(h, prec, dtype, R, A, W, out, Dl, d, Dr, wL, wR);
+}
+
+int ct_two_site_theta(ct_handle_t h, int prec, int dtype, const void* A_i, const void* A_j, void* theta, int64_t Dl,
+                      int64_t d, int64_t Dm, int64_t Dr) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(Dl > 0 && d > 0 && Dm > 0 && Dr > 0, 7, "dimensions must be > 0");
+    ct::GemmDesc g;
+    g.M = (int)(Dl * d); g.N = (int)(d * Dr); g.K = (int)Dm;
+    g.a_kc = 1; g.sAm = Dm; g.sAk = 1;
+    g.b_kc = 0; g.sBk = d * Dr; g.sBn = 1;
+    g.ldc = d * Dr;
+    return ct::gemm(h, prec, dtype, A_i, dtype, A_j, dtype, theta, g);
+}
+
+int ct_lanczos_two_site(ct_handle_t h, int prec, const double* L, const double* Wi, const double* Wj, const double* R,
+                        double* theta, int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wM, int64_t wR,
+                        int max_iter, double tol, int max_rotations, int refine, double* E0, int* num_iter,
+                        double* E_lanczos, double* residual_norm) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wM > 0 && wR > 0, 9, "dimensions must be > 0");
+    CT_ARG(max_iter >= 1 && max_iter <= 64, 15, "max_iter must be in [1, 64]");
+    ct::HeffDims hd{Dl, Dl, d, Dr, Dr, wL, wM, wR};
+    void* w2;
+    CT_CHECK(ct::ws_get(h, ct::WS_W2, (size_t)(wL * d * d * d * d * wR) * 8, &w2));
+    CT_CHECK(ct::heff_build_w2(h, CT_F64, Wi, Wj, wL, wM, wR, d, (double*)w2));
+    ct::HeffEnv env, env64;
+    CT_CHECK(ct::heff_prepare(h, prec, CT_F64, L, R, (const double*)w2, hd, &env));
+    CT_CHECK(ct::heff_prepare(h, CT_PREC_FP64, CT_F64, L, R, (const double*)w2, hd, &env64));
+    ct::LanczosResult r;
+    CT_ARG(max_rotations >= 0, 17, "max_rotations must be >= 0");
+    CT_CHECK(ct::lanczos(h, env, &env64, theta, max_iter, tol, max_rotations, refine, &r));
+    if (E0) *E0 = r.E0;
+    if (num_iter) *num_iter = r.num_iter;
+    if (E_lanczos) *E_lanczos = r.E_lanczos;
+    if (residual_norm) *residual_norm = r.res_norm;
+    return 0;
+}
+
+int ct_svd_trunc(ct_handle_t h, const double* A, int64_t m, int64_t n, int64_t chi_max, double eps, double* U,
+                 double* S, double* Vt, int64_t* chi, double* discarded_weight) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(m >= 0 && n >= 0, 3, "dimensions must be >= 0");
+    CT_ARG(chi_max >= 1, 5, "chi_max must be >= 1");
+    ct::SvdResult r;
+    CT_CHECK(ct::svd_trunc(h, A, m, n, chi_max, eps, U, S, Vt, &r));
+    if (chi) *chi = r.chi;
+    if (discarded_weight) *discarded_weight = r.discarded;
+    return 0;
+}
+
+int ct_heff_adjoint(ct_handle_t h, const double* L, const double* Wi, const double* Wj, const double* R,
+                    const double* theta, double* gWi, double* gWj, int64_t Dl, int64_t d, int64_t Dr, int64_t wL,
+                    int64_t wM, int64_t wR) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wM > 0 && wR > 0, 9, "dimensions must be > 0");
+    ct::HeffDims hd{Dl, Dl, d, Dr, Dr, wL, wM, wR};
+    return ct::heff_adjoint(h, L, Wi, Wj, R, theta, gWi, gWj, hd);
+}
+
+}  // extern "C"

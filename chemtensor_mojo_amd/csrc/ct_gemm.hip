@@ -1,0 +1,292 @@
+// MFMA GEMM kernels for gfx950 (CDNA4, wave64).
+//
+// One templated kernel serves the FP64 (v_mfma_f64_16x16x4_f64) and FP32
+// (v_mfma_f32_16x16x4_f32) paths: both MFMAs take one A and one B element per
+// lane (row/col = lane%16, k = lane/16) and return four accumulators per lane
+// (row = 4*(lane/16)+r, col = lane%16).  The 3xBF16 path (ct_gemm_bf16.hip)
+// uses v_mfma_f32_16x16x32_bf16 with a k-contiguous LDS layout.
+//
+// Block tile 128x128, K-tile 16, 256 threads = 4 waves in a 2x2 grid, each
+// wave owning a 64x64 sub-tile = 4x4 MFMA tiles.  Operand tiles are staged in
+// LDS k-major (sX[k][m]) so every fragment read is 16 consecutive elements per
+// k row.  The next K-tile is prefetched into registers while the MFMAs of the
+// current one run.  Operands may be m/n-contiguous or k-contiguous (the
+// transposes the reference performs with separate permute kernels,
+// dense_tensor.mojo:1249-1257, happen here inside the tile load).
+#include "ct_internal.h"
+
+namespace ct {
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef v4d acc_t;
+    static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef v4f acc_t;
+    static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+struct KArgs {
+    const void* A;
+    const void* B;
+    void* C;
+    int M, N, K;
+    int tilesM, tilesN;
+    int vecA, vecB;  // 1 when 16-byte vector loads are legal for the operand
+    int64_t sAm, sAk, sBk, sBn, ldc, bsA, bsB, bsC;
+    double alpha, beta;
+    const int* active;
+};
+
+constexpr int BM = 128, BN = 128, BK = 16;
+
+// Loads 4 consecutive elements (16-byte aligned when vec) and converts.
+template <typename TS, typename TD>
+__device__ __forceinline__ void load4(const TS* p, TD* o);
+template <>
+__device__ __forceinline__ void load4<double, double>(const double* p, double* o) {
+    v2d a = *(const v2d*)p, b = *(const v2d*)(p + 2);
+    o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y;
+}
+template <>
+__device__ __forceinline__ void load4<double, float>(const double* p, float* o) {
+    v2d a = *(const v2d*)p, b = *(const v2d*)(p + 2);
+    o[0] = (float)a.x; o[1] = (float)a.y; o[2] = (float)b.x; o[3] = (float)b.y;
+}
+template <>
+__device__ __forceinline__ void load4<float, float>(const float* p, float* o) {
+    v4f a = *(const v4f*)p;
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+template <>
+__device__ __forceinline__ void load4<float, double>(const float* p, double* o) {
+    v4f a = *(const v4f*)p;
+    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+}
+
+// Tile loader.  KC == false: operand X[k][x] (x contiguous, row stride sk);
+// KC == true: operand X[x][k] (k contiguous, row stride sx).  Fills 8
+// register values per thread for a BK x 128 tile at (k0, x0).
+template <typename TS, typename TC, bool KC>
+__device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, int X0, int XN, int k0, int K,
+                                          bool vec, TC* r) {
+    const int tid = threadIdx.x;
+    if (!KC) {
+        const int c = tid & 31, row = tid >> 5;  // cols 4c..4c+3, rows row and row+8
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int k = k0 + row + 8 * h;
+            const int x = X0 + 4 * c;
+            TC* o = r + 4 * h;
+            if (vec && k < K && x + 3 < XN) {
+                load4<TS, TC>(X + (int64_t)k * sk + x, o);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    o[e] = (k < K && x + e < XN) ? (TC)X[(int64_t)k * sk + (x + e)] : (TC)0;
+            }
+        }
+    } else {
+        const int x = X0 + (tid & 127), hk = tid >> 7;  // row x, k = k0+8hk .. +7
+        const int kb = k0 + 8 * hk;
+        if (vec && x < XN && kb + 7 < K) {
+            const TS* p = X + (int64_t)x * sx + kb;
+            load4<TS, TC>(p, r);
+            load4<TS, TC>(p + 4, r + 4);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                r[e] = (x < XN && kb + e < K) ? (TC)X[(int64_t)x * sx + (kb + e)] : (TC)0;
+        }
+    }
+}
+
+template <typename TC, bool KC, int LD>
+__device__ __forceinline__ void store_tile(TC (*S)[LD], const TC* r) {
+    const int tid = threadIdx.x;
+    if (!KC) {
+        const int c = tid & 31, row = tid >> 5;
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) S[row + 8 * h][4 * c + e] = r[4 * h + e];
+    } else {
+        const int x = tid & 127, hk = tid >> 7;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) S[8 * hk + e][x] = r[e];
+    }
+}
+
+template <typename TC, typename TA, typename TB, typename TO, bool AKC, bool BKC>
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
+    if (p.active && *p.active == 0) return;
+    constexpr int PAD = 4;
+    __shared__ TC sA[BK][BM + PAD];
+    __shared__ TC sB[BK][BN + PAD];
+    typedef typename Mfma<TC>::acc_t acc_t;
+
+    // XCD-aware tile order: consecutive tile ids (which share operand rows)
+    // go to the same XCD, whose L2 then serves their common tiles.
+    int bid = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    const int tm = bid % p.tilesM, tn = bid / p.tilesM;
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int z = blockIdx.z;
+    const TA* A = (const TA*)p.A + (int64_t)z * p.bsA;
+    const TB* B = (const TB*)p.B + (int64_t)z * p.bsB;
+    TO* C = (TO*)p.C + (int64_t)z * p.bsC;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    TC ra[8], rb[8];
+    const int64_t sAx = AKC ? p.sAm : 0, sAk = AKC ? 0 : p.sAk;
+    const int64_t sBx = BKC ? p.sBn : 0, sBk = BKC ? 0 : p.sBk;
+    load_tile<TA, TC, AKC>(A, sAx, sAk, m0, p.M, 0, p.K, p.vecA, ra);
+    load_tile<TB, TC, BKC>(B, sBx, sBk, n0, p.N, 0, p.K, p.vecB, rb);
+
+    for (int k0 = 0; k0 < p.K; k0 += BK) {
+        __syncthreads();
+        store_tile<TC, AKC, BM + PAD>(sA, ra);
+        store_tile<TC, BKC, BN + PAD>(sB, rb);
+        __syncthreads();
+        if (k0 + BK < p.K) {
+            load_tile<TA, TC, AKC>(A, sAx, sAk, m0, p.M, k0 + BK, p.K, p.vecA, ra);
+            load_tile<TB, TC, BKC>(B, sBx, sBk, n0, p.N, k0 + BK, p.K, p.vecB, rb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; ++kk) {
+            TC a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = sA[kk * 4 + lk][wm * 64 + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = sB[kk * 4 + lk][wn * 64 + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<TC>::op(a[i], b[j], acc[i][j]);
+        }
+    }
+
+    const double alpha = p.alpha, beta = p.beta;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + wn * 64 + j * 16 + li;
+            if (n >= p.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm * 64 + i * 16 + lk * 4 + r;
+                if (m < p.M) {
+                    TO* c = C + (int64_t)m * p.ldc + n;
+                    double v = alpha * (double)acc[i][j][r];
+                    if (beta != 0.0) v += beta * (double)(*c);
+                    *c = (TO)v;
+                }
+            }
+        }
+}
+
+template <typename T>
+bool aligned16(const void* ptr, int64_t s1, int64_t s2) {
+    constexpr int64_t q = 16 / sizeof(T);
+    return ((uintptr_t)ptr % 16 == 0) && (s1 % q == 0) && (s2 % q == 0);
+}
+
+template <typename TC, typename TA, typename TB, typename TO>
+int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDesc& g) {
+    KArgs p;
+    p.A = A; p.B = B; p.C = C;
+    p.M = g.M; p.N = g.N; p.K = g.K;
+    p.tilesM = (g.M + BM - 1) / BM;
+    p.tilesN = (g.N + BN - 1) / BN;
+    p.sAm = g.sAm; p.sAk = g.sAk; p.sBk = g.sBk; p.sBn = g.sBn;
+    p.ldc = g.ldc; p.bsA = g.bsA; p.bsB = g.bsB; p.bsC = g.bsC;
+    p.alpha = g.alpha; p.beta = g.beta; p.active = g.active;
+    const bool akc = g.a_kc != 0;
+    const bool bkc = g.b_kc != 0;
+    p.vecA = akc ? aligned16<TA>(A, g.sAm, g.bsA) : aligned16<TA>(A, g.sAk, g.bsA);
+    p.vecB = bkc ? aligned16<TB>(B, g.sBn, g.bsB) : aligned16<TB>(B, g.sBk, g.bsB);
+    dim3 grid(p.tilesM * p.tilesN, 1, g.batch), block(256);
+    if (!akc && !bkc)
+        hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, false, false>), grid, block, 0, h->stream, p);
+    else if (akc && !bkc)
+        hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, true, false>), grid, block, 0, h->stream, p);
+    else if (!akc && bkc)
+        hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, false, true>), grid, block, 0, h->stream, p);
+    else
+        hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, true, true>), grid, block, 0, h->stream, p);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename TC>
+int dispatch_io(Handle* h, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g) {
+    // operand element types: FP64 or FP32 each
+#define CT_CASE(a, b, c, TA, TB, TO) \
+    if (tA == a && tB == b && tC == c) return launch_typed<TC, TA, TB, TO>(h, A, B, C, g);
+    CT_CASE(CT_F64, CT_F64, CT_F64, double, double, double)
+    CT_CASE(CT_F32, CT_F32, CT_F32, float, float, float)
+    CT_CASE(CT_F32, CT_F64, CT_F32, float, double, float)
+    CT_CASE(CT_F32, CT_F64, CT_F64, float, double, double)
+    CT_CASE(CT_F32, CT_F32, CT_F64, float, float, double)
+    CT_CASE(CT_F64, CT_F32, CT_F32, double, float, float)
+    CT_CASE(CT_F64, CT_F64, CT_F32, double, double, float)
+    CT_CASE(CT_F64, CT_F32, CT_F64, double, float, double)
+#undef CT_CASE
+    set_last_error("gemm: unsupported operand type combination");
+    return CT_STATUS_NOT_SUPPORTED;
+}
+
+}  // namespace
+
+int gemm_bf16x3(Handle* h, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g);
+
+int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g) {
+    if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return 0;
+    if (g.K <= 0) {
+        // C = beta * C: only beta == 0 occurs in this library
+        if (g.beta != 0.0) {
+            set_last_error("gemm: K == 0 with beta != 0");
+            return CT_STATUS_NOT_SUPPORTED;
+        }
+        for (int b = 0; b < g.batch; ++b) {
+            size_t es = tC == CT_F64 ? 8 : 4;
+            char* c = (char*)C + (size_t)b * g.bsC * es;
+            CT_HIP(hipMemset2DAsync(c, (size_t)g.ldc * es, 0, (size_t)g.N * es, (size_t)g.M, h->stream));
+        }
+        return 0;
+    }
+    if (prec == CT_PREC_FP64) {
+        if (tA != CT_F64 || tB != CT_F64 || tC != CT_F64) {
+            set_last_error("FP64 contraction requires FP64 buffers");
+            return CT_STATUS_NOT_SUPPORTED;
+        }
+        return dispatch_io<double>(h, tA, A, tB, B, tC, C, g);
+    }
+    if (prec == CT_PREC_FP32) return dispatch_io<float>(h, tA, A, tB, B, tC, C, g);
+    if (prec == CT_PREC_BF16X3) return gemm_bf16x3(h, tA, A, tB, B, tC, C, g);
+    set_last_error("unknown precision");
+    return CT_STATUS_NOT_SUPPORTED;
+}
+
+}  // namespace ct

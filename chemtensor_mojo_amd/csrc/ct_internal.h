@@ -1,0 +1,235 @@
+// Internal declarations of the MI355X (gfx950) two-site DMRG library.
+// Everything here is private to libdmrg_hip.so; the public C ABI is
+// include/chemtensor_hip.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/chemtensor_hip.h"
+
+namespace ct {
+
+// Status codes (public values are in chemtensor_hip.h).
+struct Error {
+    int code;
+    std::string msg;
+};
+
+void set_last_error(const std::string& msg);
+
+#define CT_HIP(expr)                                                                    \
+    do {                                                                                \
+        hipError_t _e = (expr);                                                         \
+        if (_e != hipSuccess) {                                                         \
+            ::ct::set_last_error(std::string(#expr) + ": " + hipGetErrorString(_e));     \
+            return CT_STATUS_HIP_ERROR_BASE + (int)_e;                                  \
+        }                                                                               \
+    } while (0)
+
+// Expression form of CT_HIP: returns 0 or the library status for e.
+inline int hip_rc(hipError_t e) {
+    if (e == hipSuccess) return 0;
+    set_last_error(std::string("HIP error: ") + hipGetErrorString(e));
+    return CT_STATUS_HIP_ERROR_BASE + (int)e;
+}
+
+#define CT_CHECK(expr)                  \
+    do {                                \
+        int _s = (expr);                \
+        if (_s != 0) return _s;         \
+    } while (0)
+
+#define CT_ARG(cond, k, msg)                                          \
+    do {                                                              \
+        if (!(cond)) {                                                \
+            ::ct::set_last_error(std::string("argument ") + #k + ": " + (msg)); \
+            return -(k);                                              \
+        }                                                             \
+    } while (0)
+
+// Device scratch buffer owned by a handle.  Grows on demand (after a stream
+// sync), never shrinks until the handle is destroyed.
+struct Buffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct Handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    rocblas_handle blas = nullptr;
+    // named workspaces (H_eff intermediates, reductions, SVD, Lanczos basis ...)
+    Buffer ws[24];
+    double* d_scalars = nullptr;  // small FP64 scratch for reductions (pinned-size)
+    double* h_pinned = nullptr;   // pinned host mirror for scalar read-back
+    int num_cu = 256;
+};
+
+enum WsSlot {
+    WS_T1 = 0,      // first GEMM output
+    WS_T2 = 1,      // MPO-mixing output
+    WS_CONV_A = 2,  // operand conversions (mixed precision)
+    WS_CONV_B = 3,
+    WS_PARTIAL = 4, // reduction partial sums
+    WS_KRYLOV = 5,  // Lanczos basis
+    WS_W = 6,       // Lanczos work vector
+    WS_SVD = 7,     // SVD copies
+    WS_SVD2 = 8,
+    WS_SVD3 = 9,
+    WS_TMP = 10,
+    WS_TMP2 = 11,
+    WS_MIX = 12,    // mixing coefficient matrix
+    WS_LZ = 13,     // Lanczos device state
+    WS_TMP3 = 14,
+    WS_TMP4 = 15,
+    WS_LSPLIT = 16,  // prepared (converted / split) left environment
+    WS_RSPLIT = 17,  // prepared right environment
+    WS_VSPLIT = 18,  // split Krylov vector (3xBF16 matvec)
+    WS_USPLIT = 19,  // split mixing output (3xBF16 matvec)
+    WS_ENV_T1 = 20,  // environment update intermediates
+    WS_ENV_T2 = 21,
+    WS_ENV_CONV = 22,
+    WS_W2 = 23,      // fused MPO coefficients
+    WS_COUNT = 24
+};
+
+int ws_get(Handle* h, int slot, size_t bytes, void** out);
+
+// ---------------------------------------------------------------- GEMM
+// C[b][m, n] = alpha * sum_k A[b](m, k) * B[b](k, n) + beta * C[b][m, n]
+// A element (m, k) at A + b*bsA + m*sAm + k*sAk; exactly one of sAm, sAk is 1.
+// B element (k, n) at B + b*bsB + k*sBk + n*sBn; exactly one of sBk, sBn is 1.
+// C element (m, n) at C + b*bsC + m*ldc + n.
+struct GemmDesc {
+    int M = 0, N = 0, K = 0, batch = 1;
+    int a_kc = 0;  // 1: A stored [m][k] (sAk == 1); 0: A stored [k][m] (sAm == 1)
+    int b_kc = 0;  // 1: B stored [n][k] (sBk == 1); 0: B stored [k][n] (sBn == 1)
+    int64_t sAm = 0, sAk = 0, sBk = 0, sBn = 0, ldc = 0;
+    int64_t bsA = 0, bsB = 0, bsC = 0;
+    double alpha = 1.0, beta = 0.0;
+    const int* active = nullptr;  // optional device flag: kernel is a no-op when *active == 0
+};
+
+// dtype codes for operands: CT_F64 / CT_F32
+int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int tC, void* C,
+         const GemmDesc& g);
+
+// 3xBF16 operands pre-split into hi/lo BF16 planes with k-contiguous rows:
+// element (row, k) of batch b at hi[b*bs + row*ld + k] (same for lo).
+struct SplitOp {
+    const uint16_t* hi = nullptr;
+    const uint16_t* lo = nullptr;
+    int64_t ld = 0, bs = 0;
+};
+// Splits X (element (row, k) of batch b at X + b*bsX + row*s_row + k*s_k) into
+// hi = bf16(x), lo = bf16(x - hi) planes of layout [b][row][k] (ld_out, bs_out).
+int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
+               int64_t bsX, uint16_t* hi, uint16_t* lo, int64_t ld_out, int64_t bs_out, const int* active);
+// C[b][m][n] = alpha * sum_k (Ah Bh + Ah Bl + Al Bh)[m,k][n,k] + beta * C
+int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* C, int M, int N, int K,
+                   int64_t ldc, int64_t bsC, int batch, double alpha, double beta, const int* active);
+
+// ------------------------------------------------------------- mixing
+// Per pixel (p, q): Y[co, p, eo, q] (+)= sum_{ci, ei} coef[(ci*EIN+ei)*NOUT + co*EOUT+eo] * X[ci, p, ei, q]
+// with element strides given explicitly (q stride 1 on both sides).
+struct MixDesc {
+    int CIN = 0, EIN = 0, COUT = 0, EOUT = 0;
+    int64_t P = 0, Q = 0;
+    int64_t sXc = 0, sXp = 0, sXe = 0;
+    int64_t sYc = 0, sYp = 0, sYe = 0;
+    const int* active = nullptr;
+};
+int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef, const MixDesc& m);
+// Same, FP32 input, output written as BF16 hi/lo planes (lo plane at Y + lo_off).
+int mix_split(Handle* h, const float* X, uint16_t* Y, int64_t lo_off, const double* d_coef, const MixDesc& m);
+
+// ------------------------------------------------------------- misc kernels
+int permute(Handle* h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm, void* out);
+int convert(Handle* h, int tin, const void* in, int tout, void* out, int64_t n);
+int fill(Handle* h, int dtype, void* x, int64_t n, double value);
+int set_identity_env(Handle* h, int dtype, void* x, int64_t w, int64_t D);
+// reductions into a device FP64 scalar (no host sync)
+int dot_dev(Handle* h, int dtype, const void* x, const void* y, int64_t n, double* d_out, const int* active);
+// host-returning reductions (synchronise the stream)
+int dot_host(Handle* h, int dtype, const void* x, const void* y, int64_t n, double* out);
+int axpy(Handle* h, int dtype, double alpha, const void* x, void* y, int64_t n);
+int scale(Handle* h, int dtype, double alpha, void* x, int64_t n);
+
+// ---------------------------------------------------------- contractions
+struct HeffDims {
+    int64_t Dl, Dlb, d, Dr, Drb, wL, wM, wR;
+};
+
+// Environments prepared for one precision: FP64/FP32 keep plain pointers
+// (tEnv = element type); 3xBF16 holds the split, k-contiguous planes
+//   Ls: [wL][Dlb][Dl]  (A operand of L^T theta)
+//   Rs: [Drb][wR*Dr]   (B operand of U R)
+struct HeffEnv {
+    int prec = CT_PREC_FP64;
+    int tEnv = CT_F64;
+    const void* L = nullptr;
+    const void* R = nullptr;
+    SplitOp Ls, Rs;
+    const double* W2 = nullptr;  // fused MPO coefficients (device, FP64)
+    HeffDims dims{};
+};
+
+// Builds the fused two-site MPO coefficients
+//   coef[((wl*d + s1)*d + s2) * (d*d*wR) + wr*d*d + s1'*d + s2'] = sum_wm Wi[wl,s1,s1',wm] Wj[wm,s2,s2',wr]
+// into d_out (device FP64, wL*d*d*d*d*wR values).  W tensors are FP64 or FP32 (tW).
+int heff_build_w2(Handle* h, int tW, const void* Wi, const void* Wj, int64_t wL, int64_t wM, int64_t wR, int64_t d,
+                  double* d_out);
+// Prepares env for prec.  For 3xBF16 the split planes are written to the
+// WS_LSPLIT / WS_RSPLIT workspaces (valid until the next prepare).  For FP32
+// with FP64 input the converted copies go to the same workspaces.
+int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, const double* d_W2,
+                 const HeffDims& hd, HeffEnv* env);
+// out = H_eff theta.  tVec: element type of theta and out.
+int heff_apply(Handle* h, const HeffEnv& env, int tVec, const void* theta, void* out, const int* active);
+
+int env_left(Handle* h, int prec, int dtype, const void* L, const void* A, const void* W, void* out,
+             int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR);
+int env_right(Handle* h, int prec, int dtype, const void* R, const void* A, const void* W, void* out,
+              int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR);
+int tensordot(Handle* h, int prec, int dtype, const void* A, int rankA, const int64_t* shapeA, const void* B,
+              int rankB, const int64_t* shapeB, int ndim_mult, int axrange_A, int axrange_B, void* C, int rankC,
+              const int64_t* shapeC);
+int heff_adjoint(Handle* h, const double* L, const double* Wi, const double* Wj, const double* R,
+                 const double* theta, double* gWi, double* gWj, const HeffDims& hd);
+
+// Lanczos (device resident) ------------------------------------------------
+struct LanczosResult {
+    double E0 = 0.0;        // reported energy (refined when refine != 0)
+    double E_lanczos = 0.0; // lowest Ritz value of T (reference definition)
+    double E_rq = 0.0;      // FP64 Rayleigh quotient <psi|H|psi> (refinement)
+    double res_norm = 0.0;  // ||H psi - E_rq psi|| (refinement)
+    int num_iter = 0;
+};
+// env: environments prepared for the Krylov matvecs; env64: FP64 environments
+// for the refinement matvec (only used when refine != 0).
+int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta, int max_iter, double tol,
+            int max_rot, int refine, LanczosResult* res);
+
+// SVD / truncation --------------------------------------------------------
+struct SvdResult {
+    int64_t chi = 0;
+    double discarded = 0.0;  // (|theta|^2 - sum kept s^2) / |theta|^2
+    double norm_sq = 0.0;
+};
+// A: [m, n] row-major FP64 (device, preserved).  U: [m, chi] row-major, S[chi], Vt[chi, n].
+int svd_trunc(Handle* h, const double* A, int64_t m, int64_t n, int64_t chi_max, double eps, double* U,
+              double* S, double* Vt, SvdResult* r);
+
+}  // namespace ct
+
+// The public opaque handle type is the internal handle.
+struct ct_handle_s : public ct::Handle {};

@@ -1,0 +1,384 @@
+// Device-resident Lanczos for the two-site local problem.
+//
+// Restates lanczos_two_site_optimize (reference src/algorithms/dmrg.mojo:486-756)
+// with the Krylov basis, the recurrence coefficients and the tridiagonal
+// eigensolve on the GPU in FP64.  The host enqueues every kernel up front and
+// synchronises once at the end: the breakdown test beta < 100*dim*DBL_EPSILON
+// (:630-632, :671-673) clears a device flag that turns all later kernels of
+// the recurrence into no-ops, so no per-iteration read-back is needed (the
+// reference copies every Krylov vector device->host and back, :640-689).
+//
+// Optional refinement for mixed-precision matvecs (no reference counterpart):
+//   refine >= 1: E_rq = <psi|H psi> with an FP64 matvec of the normalised
+//                Ritz vector psi; r = H psi - E_rq psi.
+//   refine >= 2: one Rayleigh-Ritz step in span{psi, r/|r|}, whose 2x2
+//                projected matrix [[E_rq, |r|], [|r|, <r^|H r^>]] uses one more
+//                matvec at the Krylov precision; psi and the energy are updated.
+#include "ct_internal.h"
+#include "ct_device.h"
+
+namespace ct {
+
+namespace {
+
+constexpr int MAXK = 256;
+
+struct LzState {
+    double alpha[MAXK];
+    double beta[MAXK];
+    double coef[MAXK];
+    double scal[16];  // 0: |theta|^2, 1: E0 (Jacobi), 2: |ritz|^2, 3: E_rq, 4: |r|^2, 5: h_rr, 6: E_ref
+    int active;
+    int numiter;
+    int err;
+    int pad;
+};
+
+constexpr int NB = 512;  // reduction blocks
+
+__global__ void lz_init_kernel(LzState* s, int max_iter) {
+    s->active = 1;
+    s->numiter = max_iter;
+    s->err = 0;
+    for (int i = 0; i < MAXK; ++i) { s->alpha[i] = 0.0; s->beta[i] = 0.0; s->coef[i] = 0.0; }
+    for (int i = 0; i < 16; ++i) s->scal[i] = 0.0;
+}
+
+// partial sums of x.y  (y == nullptr: x.x)
+__global__ __launch_bounds__(256) void lz_dot_partial(const double* __restrict__ x, const double* __restrict__ y,
+                                                      int64_t n, double* __restrict__ part, const LzState* s,
+                                                      int gated) {
+    if (gated && !s->active) return;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = x[i];
+        acc += a * (y ? y[i] : a);
+    }
+    acc = block_sum256(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// what: 0 -> scal[idx] = sum; 1 -> alpha[idx] = sum;
+//       2 -> beta step idx: b = sqrt(sum); breakdown test; beta[idx] = b
+__global__ __launch_bounds__(256) void lz_reduce(const double* __restrict__ part, LzState* s, int what, int idx,
+                                                 double thr, int gated) {
+    if (gated && !s->active) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) acc += part[i];
+    acc = block_sum256(acc);
+    if (threadIdx.x != 0) return;
+    if (what == 0) {
+        s->scal[idx] = acc;
+    } else if (what == 1) {
+        s->alpha[idx] = acc;
+    } else {
+        const double b = sqrt(acc);
+        if (b < thr) {
+            s->numiter = idx + 1;
+            s->active = 0;
+        } else {
+            s->beta[idx] = b;
+        }
+    }
+}
+
+// v0 = theta / |theta|; zero start vector -> err (dmrg.mojo:605-613)
+__global__ __launch_bounds__(256) void lz_start(const double* __restrict__ theta, double* __restrict__ v0, int64_t n,
+                                                LzState* s) {
+    const double nrm = sqrt(s->scal[0]);
+    if (nrm < 1e-14) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { s->err = 1; s->active = 0; }
+        return;
+    }
+    const double inv = 1.0 / nrm;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v0[i] = theta[i] * inv;
+}
+
+// w -= alpha_j v_j + beta_{j-1} v_{j-1}; partial sums of w.w (dmrg.mojo:656-669)
+__global__ __launch_bounds__(256) void lz_update(double* __restrict__ w, const double* __restrict__ vj,
+                                                 const double* __restrict__ vjm1, int64_t n, int j, const LzState* s,
+                                                 double* __restrict__ part) {
+    if (!s->active) return;
+    const double a = s->alpha[j];
+    const double b = j > 0 ? s->beta[j - 1] : 0.0;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double x = w[i] - a * vj[i];
+        if (j > 0) x -= b * vjm1[i];
+        w[i] = x;
+        acc += x * x;
+    }
+    acc = block_sum256(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// v_{j+1} = w / beta_j (dmrg.mojo:677-681)
+__global__ __launch_bounds__(256) void lz_next(const double* __restrict__ w, double* __restrict__ vn, int64_t n, int j,
+                                               const LzState* s) {
+    if (!s->active) return;
+    const double inv = 1.0 / s->beta[j];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        vn[i] = w[i] * inv;
+}
+
+// Cyclic-max Jacobi on the tridiagonal T (dmrg.mojo:507-598, :709-722), one
+// wavefront.  Each of at most max_rot rotations zeroes the largest
+// off-diagonal element (first in row-major order on ties, like the serial
+// scan), stopping when it is below tol.  Writes E0 = smallest diagonal entry
+// (first on ties) and the normalised eigenvector into coef.
+__global__ __launch_bounds__(64) void lz_jacobi(LzState* s, double tol, int max_rot) {
+    __shared__ double A[64 * 64];
+    __shared__ double V[64 * 64];
+    if (s->err) return;
+    const int n = s->numiter;
+    const int lane = threadIdx.x;
+    if (n > 64) {  // the host limits max_iter; keep the kernel bounded anyway
+        if (lane == 0) s->err = 2;
+        return;
+    }
+    for (int i = lane; i < n * n; i += 64) {
+        const int r = i / n, c = i % n;
+        double v = 0.0;
+        if (r == c) v = s->alpha[r];
+        else if (c == r + 1) v = s->beta[r];
+        else if (r == c + 1) v = s->beta[c];
+        A[i] = v;
+        V[i] = r == c ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (n == 1) {
+        if (lane == 0) { s->scal[1] = A[0]; s->coef[0] = 1.0; }
+        return;
+    }
+    const int npairs = n * (n - 1) / 2;
+    for (int it = 0; it < max_rot; ++it) {
+        // argmax |A[i][j]|, i < j, first in row-major order
+        double best = -1.0;
+        int bidx = 0x7fffffff;
+        for (int t = lane; t < npairs; t += 64) {
+            // map t -> (i, j) in row-major order of the strict upper triangle
+            int i = 0, rem = t;
+            while (rem >= n - 1 - i) { rem -= n - 1 - i; ++i; }
+            const int j = i + 1 + rem;
+            const double v = fabs(A[i * n + j]);
+            const int lin = i * n + j;
+            if (v > best || (v == best && lin < bidx)) { best = v; bidx = lin; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bidx, off, 64);
+            if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+        }
+        if (best < tol) break;
+        const int p = bidx / n, q = bidx % n;
+        const double app = A[p * n + p], aqq = A[q * n + q], apq = A[p * n + q];
+        const double phi = 0.5 * atan2(2.0 * apq, aqq - app);
+        const double c = cos(phi), sn = sin(phi);
+        __syncthreads();
+        for (int k = lane; k < n; k += 64) {
+            if (k != p && k != q) {
+                const double akp = A[k * n + p], akq = A[k * n + q];
+                const double nkp = c * akp - sn * akq, nkq = sn * akp + c * akq;
+                A[k * n + p] = nkp; A[p * n + k] = nkp;
+                A[k * n + q] = nkq; A[q * n + k] = nkq;
+            }
+            const double vp = V[k * n + p], vq = V[k * n + q];
+            V[k * n + p] = c * vp - sn * vq;
+            V[k * n + q] = sn * vp + c * vq;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            A[p * n + p] = c * c * app - 2.0 * sn * c * apq + sn * sn * aqq;
+            A[q * n + q] = sn * sn * app + 2.0 * sn * c * apq + c * c * aqq;
+            A[p * n + q] = 0.0;
+            A[q * n + p] = 0.0;
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        int mi = 0;
+        double mv = A[0];
+        for (int i = 1; i < n; ++i)
+            if (A[i * n + i] < mv) { mv = A[i * n + i]; mi = i; }
+        double nr = 0.0;
+        for (int i = 0; i < n; ++i) nr += V[i * n + mi] * V[i * n + mi];
+        nr = sqrt(nr);
+        for (int i = 0; i < n; ++i) s->coef[i] = nr > 0.0 ? V[i * n + mi] / nr : V[i * n + mi];
+        s->scal[1] = mv;
+    }
+}
+
+// theta = sum_k coef[k] K[k] (k < numiter); partial sums of theta.theta (dmrg.mojo:728-742)
+__global__ __launch_bounds__(256) void lz_ritz(const double* __restrict__ K, int64_t n, double* __restrict__ theta,
+                                               const LzState* s, double* __restrict__ part) {
+    if (s->err) return;
+    const int nk = s->numiter;
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double x = 0.0;
+        for (int k = 0; k < nk; ++k) x += s->coef[k] * K[(int64_t)k * n + i];
+        theta[i] = x;
+        acc += x * x;
+    }
+    acc = block_sum256(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// x *= 1/sqrt(scal[idx]) when positive (dmrg.mojo:743-745)
+__global__ __launch_bounds__(256) void lz_normalize(double* __restrict__ x, int64_t n, const LzState* s, int idx) {
+    if (s->err) return;
+    const double nr = sqrt(s->scal[idx]);
+    if (!(nr > 0.0)) return;
+    const double inv = 1.0 / nr;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] *= inv;
+}
+
+// r = w - E_rq psi (in place in w); partial sums of r.r
+__global__ __launch_bounds__(256) void lz_residual(double* __restrict__ w, const double* __restrict__ psi, int64_t n,
+                                                   const LzState* s, double* __restrict__ part) {
+    if (s->err) return;
+    const double e = s->scal[3];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double r = w[i] - e * psi[i];
+        w[i] = r;
+        acc += r * r;
+    }
+    acc = block_sum256(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// rhat = r / |r| into out (zero when |r| == 0)
+__global__ __launch_bounds__(256) void lz_rhat(const double* __restrict__ r, double* __restrict__ out, int64_t n,
+                                               const LzState* s) {
+    if (s->err) return;
+    const double nr = sqrt(s->scal[4]);
+    const double inv = nr > 0.0 ? 1.0 / nr : 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = r[i] * inv;
+}
+
+// 2x2 Rayleigh-Ritz in span{psi, rhat}: [[E, rho], [rho, h_rr]]; lowest
+// eigenpair (c0, c1) -> psi = c0 psi + c1 rhat (normalised by construction).
+__global__ __launch_bounds__(256) void lz_correct(double* __restrict__ psi, const double* __restrict__ rhat, int64_t n,
+                                                  LzState* s) {
+    if (s->err) return;
+    const double E = s->scal[3], rho = sqrt(s->scal[4]), hrr = s->scal[5];
+    double c0 = 1.0, c1 = 0.0, lam = E;
+    if (rho > 0.0) {
+        const double mid = 0.5 * (E + hrr), half = 0.5 * (hrr - E);
+        const double rad = sqrt(half * half + rho * rho);
+        lam = mid - rad;
+        // eigenvector of [[E - lam, rho], [rho, hrr - lam]]: (rho, lam - E) up to scale
+        double x0 = rho, x1 = lam - E;
+        const double nr = sqrt(x0 * x0 + x1 * x1);
+        c0 = x0 / nr;
+        c1 = x1 / nr;
+        if (c0 < 0.0) { c0 = -c0; c1 = -c1; }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) s->scal[6] = lam;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        psi[i] = c0 * psi[i] + c1 * rhat[i];
+}
+
+inline int grid_n(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
+}
+
+}  // namespace
+
+int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta, int max_iter, double tol,
+            int max_rot, int refine, LanczosResult* res) {
+    const HeffDims& D = env.dims;
+    const int64_t dim = D.Dl * D.d * D.d * D.Dr;
+    CT_ARG(max_iter >= 1 && max_iter <= 64, 5, "max_iter must be in [1, 64]");
+    void *kb, *wb, *pb, *sb;
+    CT_CHECK(ws_get(h, WS_KRYLOV, (size_t)dim * max_iter * 8, &kb));
+    CT_CHECK(ws_get(h, WS_W, (size_t)dim * 2 * 8, &wb));
+    CT_CHECK(ws_get(h, WS_PARTIAL, NB * 8, &pb));
+    CT_CHECK(ws_get(h, WS_LZ, sizeof(LzState), &sb));
+    double* K = (double*)kb;
+    double* w = (double*)wb;
+    double* w2 = w + dim;
+    double* part = (double*)pb;
+    LzState* s = (LzState*)sb;
+    const int* act = &s->active;
+    const double thr = 100.0 * (double)dim * 2.2204460492503131e-16;
+    hipStream_t st = h->stream;
+    const int G = grid_n(dim);
+
+    hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(1), 0, st, s, max_iter);
+    hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)d_theta, (const double*)nullptr, dim,
+                       part, (const LzState*)s, 0);
+    hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 0, thr, 0);
+    hipLaunchKernelGGL(lz_start, dim3(G), dim3(256), 0, st, (const double*)d_theta, K, dim, s);
+    for (int j = 0; j < max_iter - 1; ++j) {
+        double* vj = K + (int64_t)j * dim;
+        CT_CHECK(heff_apply(h, env, CT_F64, vj, w, act));
+        hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)vj, (const double*)w, dim, part,
+                           (const LzState*)s, 1);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, j, thr, 1);
+        hipLaunchKernelGGL(lz_update, dim3(NB), dim3(256), 0, st, w, (const double*)vj,
+                           (const double*)(j > 0 ? vj - dim : vj), dim, j, (const LzState*)s, part);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 2, j, thr, 1);
+        hipLaunchKernelGGL(lz_next, dim3(G), dim3(256), 0, st, (const double*)w, vj + dim, dim, j, (const LzState*)s);
+    }
+    {   // final alpha-only step when no breakdown happened (dmrg.mojo:691-703)
+        double* vj = K + (int64_t)(max_iter - 1) * dim;
+        CT_CHECK(heff_apply(h, env, CT_F64, vj, w, act));
+        hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)vj, (const double*)w, dim, part,
+                           (const LzState*)s, 1);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, max_iter - 1, thr, 1);
+    }
+    hipLaunchKernelGGL(lz_jacobi, dim3(1), dim3(64), 0, st, s, tol, max_rot);
+    hipLaunchKernelGGL(lz_ritz, dim3(NB), dim3(256), 0, st, (const double*)K, dim, d_theta, (const LzState*)s, part);
+    hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 2, thr, 0);
+    hipLaunchKernelGGL(lz_normalize, dim3(G), dim3(256), 0, st, d_theta, dim, (const LzState*)s, 2);
+    CT_HIP(hipGetLastError());
+
+    if (refine >= 1 && env64) {
+        // E_rq = <psi|H psi> with FP64 environments
+        CT_CHECK(heff_apply(h, *env64, CT_F64, d_theta, w, nullptr));
+        hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)d_theta, (const double*)w, dim,
+                           part, (const LzState*)s, 0);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 3, thr, 0);
+        hipLaunchKernelGGL(lz_residual, dim3(NB), dim3(256), 0, st, w, (const double*)d_theta, dim, (const LzState*)s,
+                           part);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 4, thr, 0);
+        if (refine >= 2) {
+            hipLaunchKernelGGL(lz_rhat, dim3(G), dim3(256), 0, st, (const double*)w, w2, dim, (const LzState*)s);
+            // h_rr = <rhat|H rhat> at the Krylov precision (second-order effect on the energy)
+            CT_CHECK(heff_apply(h, env, CT_F64, w2, w, nullptr));
+            hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)w2, (const double*)w, dim,
+                               part, (const LzState*)s, 0);
+            hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 5, thr, 0);
+            hipLaunchKernelGGL(lz_correct, dim3(G), dim3(256), 0, st, d_theta, (const double*)w2, dim, s);
+        }
+        CT_HIP(hipGetLastError());
+    }
+
+    LzState* hs = (LzState*)h->h_pinned;
+    CT_HIP(hipMemcpyAsync(hs, s, sizeof(LzState), hipMemcpyDeviceToHost, st));
+    CT_HIP(hipStreamSynchronize(st));
+    if (hs->err == 1) {
+        set_last_error("Initial vector has zero norm");
+        return CT_STATUS_ZERO_NORM;
+    }
+    if (hs->err != 0) {
+        set_last_error("Lanczos produced an invalid Krylov basis");
+        return CT_STATUS_EMPTY_KRYLOV;
+    }
+    res->num_iter = hs->numiter;
+    res->E_lanczos = hs->scal[1];
+    res->E_rq = hs->scal[3];
+    res->res_norm = sqrt(hs->scal[4]);
+    res->E0 = res->E_lanczos;
+    if (refine >= 1 && env64) res->E0 = refine >= 2 ? hs->scal[6] : hs->scal[3];
+    return 0;
+}
+
+}  // namespace ct

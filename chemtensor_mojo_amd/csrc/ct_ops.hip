@@ -1,0 +1,331 @@
+// Memory-bound kernels of the DMRG hot path on gfx950: the MPO mixing
+// (the K = w*d contractions of apply_two_site_heff steps 2-3 and of the
+// environment updates), generic permutes, conversions, fills and FP64
+// wavefront reductions.
+#include "ct_internal.h"
+#include "ct_device.h"
+
+namespace ct {
+
+namespace {
+
+// ------------------------------------------------------------------ mixing
+// Y[co, p, eo, q] = sum_{ci, ei} coef[(ci*EIN + ei)*NOUT + co*EOUT + eo] * X[ci, p, ei, q]
+// One thread per pixel (p, q); q is the contiguous index on both sides, so
+// every load and store of a wavefront is 64 consecutive elements.
+constexpr int MIX_KMAX = 64;
+constexpr int MIX_NMAX = 64;
+
+struct MixArgs {
+    int CIN, EIN, COUT, EOUT;
+    int64_t P, Q;
+    int64_t sXc, sXp, sXe, sYc, sYp, sYe;
+    int64_t lo_off;  // split output: offset (elements) of the lo plane
+    const int* active;
+};
+
+__device__ __forceinline__ uint16_t f2bf(float x) {
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// OUT: 0 = double, 1 = float, 2 = split BF16 hi/lo planes (uint16)
+template <typename TX, typename TACC, int OUT>
+__global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void* __restrict__ Yv,
+                                                  const double* __restrict__ coef, MixArgs a) {
+    if (a.active && *a.active == 0) return;
+    __shared__ TACC sc[MIX_KMAX * MIX_NMAX];
+    const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
+    for (int i = threadIdx.x; i < K * NO; i += blockDim.x) sc[i] = (TACC)coef[i];
+    __syncthreads();
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.Q) return;
+    for (int64_t p = blockIdx.y; p < a.P; p += gridDim.y) {
+        TACC x[MIX_KMAX];
+#pragma unroll
+        for (int k = 0; k < MIX_KMAX; ++k) {
+            if (k < K) {
+                const int ci = k / a.EIN, ei = k - ci * a.EIN;
+                x[k] = (TACC)X[ci * a.sXc + p * a.sXp + ei * a.sXe + q];
+            }
+        }
+        for (int n = 0; n < NO; ++n) {
+            TACC acc = 0;
+#pragma unroll
+            for (int k = 0; k < MIX_KMAX; ++k)
+                if (k < K) acc += sc[k * NO + n] * x[k];
+            const int co = n / a.EOUT, eo = n - co * a.EOUT;
+            const int64_t o = co * a.sYc + p * a.sYp + eo * a.sYe + q;
+            if (OUT == 0) {
+                ((double*)Yv)[o] = (double)acc;
+            } else if (OUT == 1) {
+                ((float*)Yv)[o] = (float)acc;
+            } else {
+                const float f = (float)acc;
+                const uint16_t hb = f2bf(f);
+                const float hf = __uint_as_float(((uint32_t)hb) << 16);
+                ((uint16_t*)Yv)[o] = hb;
+                ((uint16_t*)Yv)[o + a.lo_off] = f2bf(f - hf);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ permute
+struct PermArgs {
+    int rank;
+    int64_t n;
+    int64_t out_shape[8];
+    int64_t in_stride_perm[8];  // stride of input axis perm[i]
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void permute_kernel(const T* __restrict__ in, T* __restrict__ out, PermArgs a) {
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < a.n; o += (int64_t)gridDim.x * blockDim.x) {
+        int64_t rem = o, src = 0;
+        for (int i = a.rank - 1; i >= 0; --i) {
+            const int64_t c = rem % a.out_shape[i];
+            rem /= a.out_shape[i];
+            src += c * a.in_stride_perm[i];
+        }
+        out[o] = in[src];
+    }
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void convert_kernel(const TI* __restrict__ in, TO* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (TO)in[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(T* __restrict__ x, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void identity_env_kernel(T* __restrict__ x, int64_t w, int64_t D) {
+    const int64_t n = w * D * D;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i % (D * D);
+        x[i] = (r / D == r % D) ? (T)1 : (T)0;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void axpy_kernel(T* __restrict__ y, const T* __restrict__ x, double alpha, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (T)((double)y[i] + alpha * (double)x[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scale_kernel(T* __restrict__ x, double alpha, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (T)(alpha * (double)x[i]);
+}
+
+}  // namespace
+
+// -------------------------------------------------------- FP64 reductions
+// Two-pass deterministic dot product: RED_BLOCKS partial sums in a fixed
+// order, then one block reduces them.  Shared with the Lanczos kernels.
+template <typename T>
+__global__ __launch_bounds__(256) void dot_partial_kernel(const T* __restrict__ x, const T* __restrict__ y, int64_t n,
+                                                          double* __restrict__ partial, const int* active) {
+    if (active && *active == 0) return;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += (double)x[i] * (double)y[i];
+    s = block_sum256(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void reduce_final_kernel(const double* __restrict__ partial, int np,
+                                                           double* __restrict__ out, const int* active) {
+    if (active && *active == 0) return;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[i];
+    s = block_sum256(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+template __global__ void dot_partial_kernel<double>(const double*, const double*, int64_t, double*, const int*);
+template __global__ void dot_partial_kernel<float>(const float*, const float*, int64_t, double*, const int*);
+
+static inline int grid_for(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef, const MixDesc& m) {
+    const int K = m.CIN * m.EIN, NO = m.COUT * m.EOUT;
+    if (K > MIX_KMAX || NO > MIX_NMAX) {
+        set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
+        return CT_STATUS_NOT_SUPPORTED;
+    }
+    if (m.P <= 0 || m.Q <= 0) return 0;
+    MixArgs a;
+    a.CIN = m.CIN; a.EIN = m.EIN; a.COUT = m.COUT; a.EOUT = m.EOUT;
+    a.P = m.P; a.Q = m.Q;
+    a.sXc = m.sXc; a.sXp = m.sXp; a.sXe = m.sXe;
+    a.sYc = m.sYc; a.sYp = m.sYp; a.sYe = m.sYe;
+    a.lo_off = 0;
+    a.active = m.active;
+    dim3 grid((unsigned)((m.Q + 255) / 256), (unsigned)(m.P < 65535 ? m.P : 65535)), block(256);
+    if (tX == CT_F64 && tY == CT_F64)
+        hipLaunchKernelGGL((mix_kernel<double, double, 0>), grid, block, 0, h->stream, (const double*)X, Y, d_coef, a);
+    else if (tX == CT_F32 && tY == CT_F32)
+        hipLaunchKernelGGL((mix_kernel<float, float, 1>), grid, block, 0, h->stream, (const float*)X, Y, d_coef, a);
+    else if (tX == CT_F32 && tY == CT_F64)
+        hipLaunchKernelGGL((mix_kernel<float, float, 0>), grid, block, 0, h->stream, (const float*)X, Y, d_coef, a);
+    else if (tX == CT_F64 && tY == CT_F32)
+        hipLaunchKernelGGL((mix_kernel<double, double, 1>), grid, block, 0, h->stream, (const double*)X, Y, d_coef, a);
+    else {
+        set_last_error("mix: unsupported types");
+        return CT_STATUS_NOT_SUPPORTED;
+    }
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+// Mixing whose output goes straight into BF16 hi/lo planes (lo plane at
+// Y + lo_off elements); X is FP32.
+int mix_split(Handle* h, const float* X, uint16_t* Y, int64_t lo_off, const double* d_coef, const MixDesc& m) {
+    const int K = m.CIN * m.EIN, NO = m.COUT * m.EOUT;
+    if (K > MIX_KMAX || NO > MIX_NMAX) {
+        set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
+        return CT_STATUS_NOT_SUPPORTED;
+    }
+    if (m.P <= 0 || m.Q <= 0) return 0;
+    MixArgs a;
+    a.CIN = m.CIN; a.EIN = m.EIN; a.COUT = m.COUT; a.EOUT = m.EOUT;
+    a.P = m.P; a.Q = m.Q;
+    a.sXc = m.sXc; a.sXp = m.sXp; a.sXe = m.sXe;
+    a.sYc = m.sYc; a.sYp = m.sYp; a.sYe = m.sYe;
+    a.lo_off = lo_off;
+    a.active = m.active;
+    dim3 grid((unsigned)((m.Q + 255) / 256), (unsigned)(m.P < 65535 ? m.P : 65535)), block(256);
+    hipLaunchKernelGGL((mix_kernel<float, float, 2>), grid, block, 0, h->stream, X, (void*)Y, d_coef, a);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+int permute(Handle* h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm, void* out) {
+    PermArgs a;
+    if (rank < 1 || rank > 8) {
+        set_last_error("permute: rank must be 1..8");
+        return CT_STATUS_NOT_SUPPORTED;
+    }
+    int64_t stride[8];
+    stride[rank - 1] = 1;
+    for (int i = rank - 2; i >= 0; --i) stride[i] = stride[i + 1] * shape[i + 1];
+    a.rank = rank;
+    a.n = 1;
+    for (int i = 0; i < rank; ++i) {
+        a.out_shape[i] = shape[perm[i]];
+        a.in_stride_perm[i] = stride[perm[i]];
+        a.n *= shape[i];
+    }
+    if (a.n == 0) return 0;
+    if (dtype == CT_F64)
+        hipLaunchKernelGGL(permute_kernel<double>, dim3(grid_for(a.n)), dim3(256), 0, h->stream, (const double*)in,
+                           (double*)out, a);
+    else
+        hipLaunchKernelGGL(permute_kernel<float>, dim3(grid_for(a.n)), dim3(256), 0, h->stream, (const float*)in,
+                           (float*)out, a);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+int convert(Handle* h, int tin, const void* in, int tout, void* out, int64_t n) {
+    if (n <= 0) return 0;
+    dim3 g(grid_for(n)), b(256);
+    if (tin == CT_F64 && tout == CT_F32)
+        hipLaunchKernelGGL((convert_kernel<double, float>), g, b, 0, h->stream, (const double*)in, (float*)out, n);
+    else if (tin == CT_F32 && tout == CT_F64)
+        hipLaunchKernelGGL((convert_kernel<float, double>), g, b, 0, h->stream, (const float*)in, (double*)out, n);
+    else if (tin == tout) {
+        CT_HIP(hipMemcpyAsync(out, in, (size_t)n * (tin == CT_F64 ? 8 : 4), hipMemcpyDeviceToDevice, h->stream));
+        return 0;
+    } else {
+        set_last_error("convert: unsupported types");
+        return CT_STATUS_NOT_SUPPORTED;
+    }
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+int fill(Handle* h, int dtype, void* x, int64_t n, double value) {
+    if (n <= 0) return 0;
+    if (dtype == CT_F64)
+        hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, (double*)x, n, value);
+    else
+        hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, (float*)x, n, (float)value);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+int set_identity_env(Handle* h, int dtype, void* x, int64_t w, int64_t D) {
+    const int64_t n = w * D * D;
+    if (n <= 0) return 0;
+    if (dtype == CT_F64)
+        hipLaunchKernelGGL(identity_env_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, (double*)x, w, D);
+    else
+        hipLaunchKernelGGL(identity_env_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, (float*)x, w, D);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+constexpr int RED_BLOCKS = 512;
+
+int dot_dev(Handle* h, int dtype, const void* x, const void* y, int64_t n, double* d_out, const int* active) {
+    void* part;
+    CT_CHECK(ws_get(h, WS_PARTIAL, RED_BLOCKS * sizeof(double), &part));
+    if (dtype == CT_F64)
+        hipLaunchKernelGGL(dot_partial_kernel<double>, dim3(RED_BLOCKS), dim3(256), 0, h->stream, (const double*)x,
+                           (const double*)y, n, (double*)part, active);
+    else
+        hipLaunchKernelGGL(dot_partial_kernel<float>, dim3(RED_BLOCKS), dim3(256), 0, h->stream, (const float*)x,
+                           (const float*)y, n, (double*)part, active);
+    hipLaunchKernelGGL(reduce_final_kernel, dim3(1), dim3(256), 0, h->stream, (const double*)part, RED_BLOCKS, d_out,
+                       active);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+int dot_host(Handle* h, int dtype, const void* x, const void* y, int64_t n, double* out) {
+    CT_CHECK(dot_dev(h, dtype, x, y, n, h->d_scalars, nullptr));
+    CT_HIP(hipMemcpyAsync(h->h_pinned, h->d_scalars, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    CT_HIP(hipStreamSynchronize(h->stream));
+    *out = h->h_pinned[0];
+    return 0;
+}
+
+int axpy(Handle* h, int dtype, double alpha, const void* x, void* y, int64_t n) {
+    if (n <= 0) return 0;
+    if (dtype == CT_F64)
+        hipLaunchKernelGGL(axpy_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, (double*)y,
+                           (const double*)x, alpha, n);
+    else
+        hipLaunchKernelGGL(axpy_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, (float*)y,
+                           (const float*)x, alpha, n);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+int scale(Handle* h, int dtype, double alpha, void* x, int64_t n) {
+    if (n <= 0) return 0;
+    if (dtype == CT_F64)
+        hipLaunchKernelGGL(scale_kernel<double>, dim3(grid_for(n)), dim3(256), 0, h->stream, (double*)x, alpha, n);
+    else
+        hipLaunchKernelGGL(scale_kernel<float>, dim3(grid_for(n)), dim3(256), 0, h->stream, (float*)x, alpha, n);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace ct

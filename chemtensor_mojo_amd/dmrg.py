@@ -1,0 +1,260 @@
+"""Two-site DMRG call surface, mirroring src/algorithms/dmrg.mojo.
+
+``dmrg_two_site`` runs the whole sweep schedule inside libdmrg_hip.so (C++
+driver, every tensor resident in HBM); the per-step functions
+(``apply_two_site_heff``, ``lanczos_two_site_optimize``,
+``split_two_site_theta_svd``, ...) expose the same kernels one call at a time
+with the reference's signatures.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import PREC_BF16X3, PREC_FP32, PREC_FP64, call
+from .dense_tensor import DenseTensor, DeviceContext, default_prec
+from .state import MatrixProductOperator, MatrixProductState, MPOSite, MPSSite
+
+
+@dataclass
+class DMRGParams:
+    """DMRGParams (dmrg.mojo:22-36) plus the precision controls of this build.
+
+    prec            precision of the Krylov matvecs (PREC_FP64 / PREC_FP32 / PREC_BF16X3)
+    refine          0: reference energies; 1: FP64 Rayleigh quotient of each local
+                    Ritz vector; 2: plus one residual-correction step
+    env_prec        precision of the environment updates
+    jacobi_max_rot  rotation cap of the Jacobi eigensolve of T; 200 is the
+                    reference's value (dmrg.mojo:510), which leaves 20x20 T
+                    matrices unconverged late in a run
+    """
+    num_sweeps: int
+    chi_max: int
+    eps_trunc: float
+    max_krylov_iter: int
+    krylov_tol: float
+    energy_tol: float
+    two_site: bool = True
+    verbose: bool = False
+    prec: int = PREC_FP64
+    refine: int = 0
+    env_prec: int = PREC_FP64
+    jacobi_max_rot: int = 200
+
+
+@dataclass
+class DMRGStats:
+    energy: float = 0.0
+    sweeps: int = 0
+    local_updates: int = 0
+    trunc_weight_sum: float = 0.0
+    trunc_weight_max: float = 0.0
+    max_bond: int = 0
+    t_total: float = 0.0
+    t_lanczos: float = 0.0
+    t_svd: float = 0.0
+    t_env: float = 0.0
+    t_theta: float = 0.0
+    matvecs: int = 0
+    matvec_flops: float = 0.0
+    env_flops: float = 0.0
+    sweep_energies: list = field(default_factory=list)
+
+
+def build_two_site_theta(A_i: MPSSite, A_ip1: MPSSite, ctx: DeviceContext) -> DenseTensor:
+    """theta[Dl,d,d,Dr] = A_i[Dl*d,Dm] @ A_{i+1}[Dm,d*Dr] (dmrg.mojo:93-140)."""
+    Dl, d, Dm = A_i.tensor.shape
+    Dm2, d2, Dr = A_ip1.tensor.shape
+    if Dm2 != Dm:
+        raise ValueError("Bond dimension mismatch in build_two_site_theta")
+    if d2 != d:
+        raise ValueError("Physical dimensions must match")
+    t = A_i.tensor
+    out = torch.empty(Dl * d * d * Dr, dtype=t.dtype, device=t.storage.device)
+    call("ct_two_site_theta", ctx.handle, default_prec(t.dtype), t.dtype_code, t.ptr(), A_ip1.tensor.ptr(),
+         ctypes.c_void_p(out.data_ptr()), Dl, d, Dm, Dr)
+    return DenseTensor(out, [Dl, d, d, Dr])
+
+
+def apply_two_site_heff(theta: DenseTensor, L_env: DenseTensor, R_env: DenseTensor, W_i: MPOSite, W_ip1: MPOSite,
+                        ctx: DeviceContext, prec: int | None = None) -> DenseTensor:
+    """H_eff|theta> without forming H_eff (dmrg.mojo:254-374)."""
+    Dl, d, _, Dr = theta.shape
+    wL, Dl2, Dlb = L_env.shape
+    wR, Dr2, Drb = R_env.shape
+    wM = W_i.tensor.shape[3]
+    if Dl2 != Dl or Dr2 != Dr or W_i.tensor.shape[0] != wL or W_ip1.tensor.shape[3] != wR \
+            or W_ip1.tensor.shape[0] != wM:
+        raise ValueError("apply_two_site_heff: shape mismatch")
+    dt = theta.dtype
+    out = torch.empty(Dlb * d * d * Drb, dtype=dt, device=theta.storage.device)
+    p = default_prec(dt) if prec is None else prec
+    call("ct_heff_apply", ctx.handle, p, theta.dtype_code, L_env.ptr(), W_i.tensor.ptr(), W_ip1.tensor.ptr(),
+         R_env.ptr(), theta.ptr(), ctypes.c_void_p(out.data_ptr()), Dl, Dlb, d, Dr, Drb, wL, wM, wR)
+    return DenseTensor(out, [Dlb, d, d, Drb])
+
+
+def lanczos_two_site_optimize(initial_theta: DenseTensor, L_env: DenseTensor, R_env: DenseTensor, W_i: MPOSite,
+                              W_ip1: MPOSite, ctx: DeviceContext, max_iter: int, tol: float,
+                              prec: int = PREC_FP64, refine: int = 0, return_info: bool = False,
+                              max_rotations: int = 200):
+    """Lowest eigenpair of the local H_eff (dmrg.mojo:486-756), FP64 Krylov basis on
+    the GPU.  Returns (E0, theta*) like the reference; with return_info also a dict
+    with the Krylov dimension, the raw Lanczos value and the residual norm."""
+    for t in (initial_theta, L_env, R_env, W_i.tensor, W_ip1.tensor):
+        if t.dtype != torch.float64:
+            raise TypeError("lanczos_two_site_optimize works on FP64 tensors (prec selects the matvec precision)")
+    Dl, d, _, Dr = initial_theta.shape
+    wL, wM, wR = L_env.shape[0], W_i.tensor.shape[3], R_env.shape[0]
+    th = initial_theta.storage.clone()
+    E = ctypes.c_double()
+    n = ctypes.c_int()
+    El = ctypes.c_double()
+    rn = ctypes.c_double()
+    call("ct_lanczos_two_site", ctx.handle, int(prec), L_env.ptr(), W_i.tensor.ptr(), W_ip1.tensor.ptr(),
+         R_env.ptr(), ctypes.c_void_p(th.data_ptr()), Dl, d, Dr, wL, wM, wR, int(max_iter), float(tol),
+         int(max_rotations), int(refine), ctypes.byref(E), ctypes.byref(n), ctypes.byref(El), ctypes.byref(rn))
+    out = DenseTensor(th, [Dl, d, d, Dr])
+    if return_info:
+        return E.value, out, {"num_iter": n.value, "E_lanczos": El.value, "residual_norm": rn.value}
+    return E.value, out
+
+
+def dense_tensor_svd_trunc(tensor: DenseTensor, ctx: DeviceContext, chi_max: int, eps_trunc: float = 1e-12):
+    """Truncated SVD of a [m, n] FP64 matrix on the GPU (dense_tensor.mojo:1483-1623).
+    Returns (U [m,chi], S [chi], Vt [chi,n], chi, discarded_weight)."""
+    if tensor.rank() != 2:
+        raise ValueError("dense_tensor_svd_trunc expects a matrix")
+    if tensor.dtype != torch.float64:
+        raise TypeError("dense_tensor_svd_trunc works in FP64")
+    m, n = tensor.shape
+    k = min(m, n, chi_max)
+    dev = tensor.storage.device
+    U = torch.empty(max(m * k, 1), dtype=torch.float64, device=dev)
+    S = torch.empty(max(k, 1), dtype=torch.float64, device=dev)
+    Vt = torch.empty(max(k * n, 1), dtype=torch.float64, device=dev)
+    chi = ctypes.c_int64()
+    dw = ctypes.c_double()
+    call("ct_svd_trunc", ctx.handle, tensor.ptr(), m, n, chi_max, float(eps_trunc), ctypes.c_void_p(U.data_ptr()),
+         ctypes.c_void_p(S.data_ptr()), ctypes.c_void_p(Vt.data_ptr()), ctypes.byref(chi), ctypes.byref(dw))
+    c = chi.value
+    if c == 0:
+        return None, None, None, 0, dw.value
+    return (DenseTensor(U[: m * c], [m, c]), DenseTensor(S[:c], [c]), DenseTensor(Vt[: c * n], [c, n]), c,
+            dw.value)
+
+
+def split_two_site_theta_svd(theta: DenseTensor, chi_max: int, eps_trunc: float, ctx: DeviceContext,
+                             left_to_right: bool = True):
+    """theta[Dl,d,d,Dr] -> (A_i[Dl,d,chi], A_{i+1}[chi,d,Dr], discarded_weight)
+    (dmrg.mojo:143-252): S goes right on L->R, left on R->L."""
+    Dl, d, _, Dr = theta.shape
+    U, S, Vt, chi, dw = dense_tensor_svd_trunc(theta.reshape([Dl * d, d * Dr]), ctx, chi_max, eps_trunc)
+    if chi == 0:
+        raise RuntimeError("truncation kept no singular value")
+    s = S.storage
+    if left_to_right:
+        A_j = (Vt.storage.view(chi, d * Dr) * s[:, None]).reshape(-1)
+        A_i = U.storage
+    else:
+        A_i = (U.storage.view(Dl * d, chi) * s[None, :]).reshape(-1)
+        A_j = Vt.storage
+    return MPSSite(DenseTensor(A_i.contiguous(), [Dl, d, chi])), MPSSite(DenseTensor(A_j.contiguous(), [chi, d, Dr])), dw
+
+
+class DMRGRun:
+    """Owns a library-side DMRG state (MPS, environments, workspaces on the device)."""
+
+    def __init__(self, ctx: DeviceContext, mpo: MatrixProductOperator, mps: MatrixProductState, params: DMRGParams):
+        if not params.two_site:
+            raise ValueError("Only two-site DMRG is currently implemented")
+        N = mps.num_sites()
+        if N < 2:
+            raise ValueError("DMRG requires at least 2 sites")
+        if mpo.num_sites() != N:
+            raise ValueError("MPS and MPO must have the same number of sites")
+        self.ctx = ctx
+        self.N = N
+        self.d = mps.physical_dim
+        self.params = params
+        # FP64 device copies (the driver keeps FP64 tensors; prec selects the matvec precision)
+        self._mpo = [s.tensor.storage.to(torch.float64) for s in mpo.sites]
+        self._mps = [s.tensor.storage.to(torch.float64) for s in mps.sites]
+        p = _lib.DMRGParamsC(params.num_sweeps, params.chi_max, params.eps_trunc, params.max_krylov_iter,
+                             params.krylov_tol, params.energy_tol, params.jacobi_max_rot, params.prec, params.refine,
+                             params.env_prec, int(params.verbose))
+        wd = _lib.i64_array(mpo.bond_dims)
+        md = _lib.i64_array(mps.bond_dims)
+        wp = (ctypes.c_void_p * N)(*[t.data_ptr() for t in self._mpo])
+        mp = (ctypes.c_void_p * N)(*[t.data_ptr() for t in self._mps])
+        st = ctypes.c_void_p()
+        call("ct_dmrg_create", ctx.handle, ctypes.byref(p), N, self.d, wd, wp, md, mp, ctypes.byref(st))
+        self.state = st
+
+    def run(self, num_sweeps: int = 0) -> tuple[float, list[float]]:
+        n = num_sweeps if num_sweeps > 0 else self.params.num_sweeps
+        E = ctypes.c_double()
+        se = (ctypes.c_double * max(n, 1))()
+        call("ct_dmrg_run", self.state, n, ctypes.byref(E), se)
+        st = self.stats()
+        return E.value, list(se)[: max(0, min(n, st.sweeps))]
+
+    def stats(self) -> DMRGStats:
+        c = _lib.DMRGStatsC()
+        call("ct_dmrg_get_stats", self.state, ctypes.byref(c))
+        return DMRGStats(**{k: getattr(c, k) for k, _ in c._fields_})
+
+    def dims(self) -> list[int]:
+        d = (ctypes.c_int64 * (self.N + 1))()
+        call("ct_dmrg_get_dims", self.state, d)
+        return list(d)
+
+    def mps(self) -> MatrixProductState:
+        dims = self.dims()
+        sites = []
+        for i in range(self.N):
+            n = dims[i] * self.d * dims[i + 1]
+            t = torch.empty(n, dtype=torch.float64, device=self.ctx.device)
+            call("ct_dmrg_get_site", self.state, i, ctypes.c_void_p(t.data_ptr()))
+            sites.append(MPSSite(DenseTensor(t, [dims[i], self.d, dims[i + 1]])))
+        return MatrixProductState(sites)
+
+    def close(self) -> None:
+        if self.state is not None and self.state.value:
+            _lib.load().ct_dmrg_destroy(self.state)
+            self.state = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dmrg_two_site(ctx: DeviceContext, mpo: MatrixProductOperator, mps: MatrixProductState, params: DMRGParams,
+                  return_stats: bool = False):
+    """Two-site DMRG (dmrg.mojo:377-483).  Returns (energy, optimised MPS) like the
+    reference; the energy is the last local Lanczos eigenvalue (:458-463), or its
+    FP64-refined value when params.refine != 0."""
+    run = DMRGRun(ctx, mpo, mps, params)
+    try:
+        E, sweep_E = run.run()
+        out = run.mps()
+        st = run.stats()
+        st.sweep_energies = sweep_E
+    finally:
+        run.close()
+    if return_stats:
+        return E, out, st
+    return E, out
+
+
+__all__ = [
+    "DMRGParams", "DMRGStats", "DMRGRun", "dmrg_two_site", "apply_two_site_heff", "lanczos_two_site_optimize",
+    "build_two_site_theta", "split_two_site_theta_svd", "dense_tensor_svd_trunc", "PREC_FP64", "PREC_FP32",
+    "PREC_BF16X3",
+]

@@ -1,0 +1,228 @@
+/*
+ * chemtensor_hip.h - C ABI of libdmrg_hip.so, the MI355X (gfx950) two-site
+ * DMRG hot path: dense tensordot, the two-site effective-Hamiltonian matvec,
+ * the left/right environment updates, the device-resident Lanczos solver,
+ * the truncated two-site SVD and the sweep driver.
+ *
+ * The library is the sibling of the reference's only native component,
+ * native/svd_shim.c, and follows its conventions (native/svd_shim.c:7-23):
+ *   - plain extern "C" functions returning an int status;
+ *   - the caller allocates every input and output buffer (here: device memory
+ *     of the handle's device, row-major contiguous, int64_t dimensions);
+ *   - the library never frees caller memory.
+ *
+ * Status: 0 = ok; -k = the k-th argument is invalid (LAPACK style, as the
+ * reference's LAPACKE `info`); CT_STATUS_* > 0 for runtime failures.
+ * ct_last_error() returns a message for the calling thread's last failure.
+ *
+ * All calls are asynchronous on the handle's stream unless they return a host
+ * scalar (documented per call), in which case they synchronise that stream.
+ * A handle is not thread-safe.
+ */
+#ifndef CHEMTENSOR_HIP_H
+#define CHEMTENSOR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums ----------------------------------------------------------- */
+/* compute precision of the contractions */
+enum {
+    CT_PREC_FP64 = 0,   /* FP64 MFMA, FP64 accumulate (the oracle-equivalent path) */
+    CT_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulate                              */
+    CT_PREC_BF16X3 = 2  /* a*b ~ ah*bh + ah*bl + al*bh on BF16 MFMA, FP32 accumulate */
+};
+/* element type of buffers */
+enum { CT_F64 = 0, CT_F32 = 1 };
+
+enum {
+    CT_STATUS_OK = 0,
+    CT_STATUS_NOT_SUPPORTED = 1,
+    CT_STATUS_ZERO_NORM = 2,        /* Lanczos start vector has zero norm (dmrg.mojo:610-611) */
+    CT_STATUS_EMPTY_KRYLOV = 3,     /* (dmrg.mojo:706-707) */
+    CT_STATUS_SVD_FAILED = 4,       /* rocSOLVER info != 0 (src/mylinalg/svd.mojo:75-76) */
+    CT_STATUS_ALLOC = 5,
+    CT_STATUS_SHAPE = 6,            /* "Dim mismatch" / wrong C shape (dense_tensor.mojo:1163-1167, 1294-1299) */
+    CT_STATUS_ROCBLAS_ERROR_BASE = 2000,
+    CT_STATUS_HIP_ERROR_BASE = 1000
+};
+
+typedef struct ct_handle_s* ct_handle_t;
+typedef struct ct_dmrg_s* ct_dmrg_t;
+
+/* ---- handle ------------------------------------------------------------
+ * Replaces the reference's DeviceContext for this path (src/algorithms/dmrg.mojo:378).
+ * stream == NULL creates a library-owned stream (default priority). */
+int ct_handle_create(int device, void* stream, ct_handle_t* out);
+int ct_handle_destroy(ct_handle_t h);
+int ct_handle_set_stream(ct_handle_t h, void* stream);
+int ct_synchronize(ct_handle_t h);
+const char* ct_last_error(void);
+const char* ct_version(void);
+
+/* ---- dense tensor core ------------------------------------------------ */
+
+/* dense_tensor_dot (src/m_tensor/dense_tensor.mojo:963-1331): contracts
+ * ndim_mult leading (axrange=1) or trailing (axrange=0) axes of A and B
+ * (aligned, not reversed); C = non-contracted A axes then B axes.  With
+ * axrange_A == axrange_B == 0 the B side is inferred exactly as :1141-1159.
+ * rankC == 0, or rankC == 1 with shapeC[0] == 1, accepts a scalar result.
+ * dtype: element type of A, B and C. */
+int ct_tensordot(ct_handle_t h, int prec, int dtype, const void* A, int rankA, const int64_t* shapeA,
+                 const void* B, int rankB, const int64_t* shapeB, int ndim_mult, int axrange_A,
+                 int axrange_B, void* C, int rankC, const int64_t* shapeC);
+
+/* 3xBF16 building blocks (no reference counterpart: the reference GEMM is
+ * float32 only, dense_tensor.mojo:1320-1328).  ct_split_bf16 writes
+ * hi = bf16(x) and lo = bf16(x - hi) of the matrix X (element (r, k) at
+ * X[r*s_row + k*s_k], dtype CT_F64/CT_F32) as k-contiguous rows of length
+ * ld >= K (uint16 bit patterns).  ct_gemm_bf16x3_nt computes
+ * C[m][n] = sum_k Ah Bh + Ah Bl + Al Bh over A [M][K] and B [N][K] split
+ * planes on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; C is CT_F64 or
+ * CT_F32 with row stride ldc. */
+int ct_split_bf16(ct_handle_t h, int dtype, const void* X, int64_t rows, int64_t K, int64_t s_row, int64_t s_k,
+                  uint16_t* hi, uint16_t* lo, int64_t ld);
+int ct_gemm_bf16x3_nt(ct_handle_t h, const uint16_t* A_hi, const uint16_t* A_lo, int64_t lda, const uint16_t* B_hi,
+                      const uint16_t* B_lo, int64_t ldb, int dtype_c, void* C, int64_t M, int64_t N, int64_t K,
+                      int64_t ldc);
+
+/* DenseTensor.transpose (dense_tensor.mojo:460-545): out axis i = in axis perm[i]; rank <= 8. */
+int ct_permute(ct_handle_t h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm,
+               void* out);
+
+/* BLAS-1 of DenseTensor (dense_tensor.mojo:637-754), FP64 accumulation.
+ * ct_norm_sq / ct_dot synchronise and return a host double. */
+int ct_norm_sq(ct_handle_t h, int dtype, const void* x, int64_t n, double* result);
+int ct_dot(ct_handle_t h, int dtype, const void* x, const void* y, int64_t n, double* result);
+int ct_axpy(ct_handle_t h, int dtype, double alpha, const void* x, void* y, int64_t n);
+int ct_scale(ct_handle_t h, int dtype, double alpha, void* x, int64_t n);
+int ct_convert(ct_handle_t h, int dtype_in, const void* in, int dtype_out, void* out, int64_t n);
+
+/* ---- two-site DMRG contractions ----------------------------------------- */
+
+/* apply_two_site_heff (src/algorithms/dmrg.mojo:254-374):
+ *   out[a',s1',s2',b'] = sum L[wl,a,a'] theta[a,s1,s2,b] Wi[wl,s1,s1',wm] Wj[wm,s2,s2',wr] R[wr,b,b']
+ * L [wL,Dl,Dlb], R [wR,Dr,Drb], Wi [wL,d,d,wM], Wj [wM,d,d,wR], theta [Dl,d,d,Dr],
+ * out [Dlb,d,d,Drb]; all of element type dtype. */
+int ct_heff_apply(ct_handle_t h, int prec, int dtype, const void* L, const void* Wi, const void* Wj,
+                  const void* R, const void* theta, void* out, int64_t Dl, int64_t Dlb, int64_t d, int64_t Dr,
+                  int64_t Drb, int64_t wL, int64_t wM, int64_t wR);
+
+/* update_left_environment (src/state/environments.mojo:99-201):
+ *   out[wR,Dr,Dr'] = sum L[wL,Dl,Dl'] A[Dl,s,Dr] W[wL,s,s',wR] A[Dl',s',Dr']
+ * L [wL,Dl,Dl], A [Dl,d,Dr], W [wL,d,d,wR], out [wR,Dr,Dr]. */
+int ct_env_update_left(ct_handle_t h, int prec, int dtype, const void* L, const void* A, const void* W,
+                       void* out, int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR);
+
+/* update_right_environment (src/state/environments.mojo:309-417):
+ *   out[wL,Dl,Dl'] = sum R[wR,Dr,Dr'] A[Dl,s,Dr] W[wL,s,s',wR] A[Dl',s',Dr']
+ * R [wR,Dr,Dr], A [Dl,d,Dr], W [wL,d,d,wR], out [wL,Dl,Dl]. */
+int ct_env_update_right(ct_handle_t h, int prec, int dtype, const void* R, const void* A, const void* W,
+                        void* out, int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR);
+
+/* build_two_site_theta (src/algorithms/dmrg.mojo:93-140): theta[Dl,d,d,Dr] = A_i[Dl*d,Dm] @ A_j[Dm,d*Dr]. */
+int ct_two_site_theta(ct_handle_t h, int prec, int dtype, const void* A_i, const void* A_j, void* theta,
+                      int64_t Dl, int64_t d, int64_t Dm, int64_t Dr);
+
+/* lanczos_two_site_optimize (src/algorithms/dmrg.mojo:486-756), FP64 Krylov
+ * basis on the device, reference variant (no reorthogonalisation, breakdown
+ * at beta < 100*dim*DBL_EPSILON, cyclic-max Jacobi on T with tolerance tol and
+ * at most max_rotations rotations - the reference's cap is 200, dmrg.mojo:510 -,
+ * normalised FP64 Ritz vector).  theta (FP64, [Dl,d,d,Dr]) is the start
+ * vector on entry and the Ritz vector on return.
+ * L, R, Wi, Wj are FP64.  prec selects the matvec precision.  refine != 0 adds
+ * an FP64 Rayleigh quotient of the Ritz vector (E_rq) and, refine == 2, one
+ * residual-correction step in span{psi, H psi - E_rq psi}; E0 then returns the
+ * refined energy.  Synchronises the stream.  num_iter = Krylov dimension. */
+int ct_lanczos_two_site(ct_handle_t h, int prec, const double* L, const double* Wi, const double* Wj,
+                        const double* R, double* theta, int64_t Dl, int64_t d, int64_t Dr, int64_t wL,
+                        int64_t wM, int64_t wR, int max_iter, double tol, int max_rotations, int refine,
+                        double* E0, int* num_iter, double* E_lanczos, double* residual_norm);
+
+/* dense_tensor_svd_trunc (src/m_tensor/dense_tensor.mojo:1483-1623) on the
+ * device in FP64 (rocSOLVER): A [m,n] (preserved), U [m,chi], S [chi],
+ * Vt [chi,n] with chi = smallest chi <= chi_max whose discarded weight
+ * (sum s^2 - sum_{<chi} s^2)/sum s^2 <= eps, else min(m,n,chi_max)
+ * (:1528-1548).  U/S/Vt must hold min(m,n,chi_max) columns/values/rows.
+ * Also returns the discarded weight the reference computes and drops
+ * (dmrg.mojo:188-200, :824).  Synchronises the stream. */
+int ct_svd_trunc(ct_handle_t h, const double* A, int64_t m, int64_t n, int64_t chi_max, double eps,
+                 double* U, double* S, double* Vt, int64_t* chi, double* discarded_weight);
+
+/* ---- sweep driver ------------------------------------------------------
+ * dmrg_two_site (src/algorithms/dmrg.mojo:377-483) with DMRGWorkspace
+ * (:39-90) and dmrg_local_update_two_site (:759-844).  The MPS and MPO are
+ * copied into library-owned device storage at create time; ct_dmrg_run may be
+ * called repeatedly (it continues from the current MPS, rebuilding the right
+ * environments as the reference does on every dmrg_two_site call). */
+typedef struct {
+    int num_sweeps;        /* DMRGParams.num_sweeps */
+    int64_t chi_max;       /* DMRGParams.chi_max */
+    double eps_trunc;      /* DMRGParams.eps_trunc */
+    int max_krylov_iter;   /* DMRGParams.max_krylov_iter */
+    double krylov_tol;     /* DMRGParams.krylov_tol (Jacobi tolerance) */
+    double energy_tol;     /* DMRGParams.energy_tol */
+    int jacobi_max_rot;    /* Jacobi rotation cap of the local eigensolve (reference: 200, dmrg.mojo:510) */
+    int prec;              /* matvec precision CT_PREC_* */
+    int refine;            /* 0: reference energies; 1: FP64 Rayleigh quotient; 2: + residual step */
+    int env_prec;          /* precision of the environment updates (CT_PREC_*) */
+    int verbose;
+} ct_dmrg_params;
+
+typedef struct {
+    double energy;             /* final energy (last local energy, dmrg.mojo:458-463) */
+    int sweeps;                /* sweeps completed */
+    int local_updates;
+    double trunc_weight_sum;   /* sum over local updates of the discarded weight */
+    double trunc_weight_max;
+    int64_t max_bond;          /* largest bond dimension reached */
+    double t_total;            /* seconds (host wall clock) */
+    double t_lanczos;          /* seconds in the Lanczos solves (incl. matvecs) */
+    double t_svd;              /* seconds in the SVD/truncation */
+    double t_env;              /* seconds in environment updates */
+    double t_theta;            /* seconds forming theta */
+    int64_t matvecs;           /* number of H_eff applications */
+    double matvec_flops;       /* algorithmic FLOPs of those matvecs (reference order) */
+    double env_flops;
+} ct_dmrg_stats;
+
+/* mpo_dims[nsites+1], mpo_sites[i]: device FP64 [w_i,d,d,w_{i+1}];
+ * mps_dims[nsites+1], mps_sites[i]: device FP64 [D_i,d,D_{i+1}]. */
+int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int64_t d, const int64_t* mpo_dims,
+                   const double* const* mpo_sites, const int64_t* mps_dims, const double* const* mps_sites,
+                   ct_dmrg_t* out);
+int ct_dmrg_destroy(ct_dmrg_t s);
+/* Runs params.num_sweeps (or num_sweeps if > 0) sweeps; sweep_energies (host,
+ * may be NULL) receives one energy per sweep.  Synchronises. */
+int ct_dmrg_run(ct_dmrg_t s, int num_sweeps, double* energy, double* sweep_energies);
+int ct_dmrg_get_stats(ct_dmrg_t s, ct_dmrg_stats* stats);
+/* current bond dimensions (nsites+1 values) */
+int ct_dmrg_get_dims(ct_dmrg_t s, int64_t* dims);
+/* copies site i ([D_i,d,D_{i+1}] FP64) into a device buffer */
+int ct_dmrg_get_site(ct_dmrg_t s, int i, double* dst);
+
+/* ---- adjoint (no reference counterpart: new capability) -----------------
+ * dE/dW_i of E = <psi|H|psi> for a two-site wave function theta at sites
+ * (i, i+1) with environments L [wL,Dl,Dl], R [wR,Dr,Dr]:
+ *   gWi[wl,s1,s1',wm] = sum L[wl,a,a'] theta[a,s1,s2,b] Wj[wm,s2,s2',wr] R[wr,b,b'] theta[a',s1',s2',b']
+ *   gWj[wm,s2,s2',wr] = sum L theta Wi R theta   (same network, Wj removed)
+ * FP64.  Either gradient pointer may be NULL. */
+int ct_heff_adjoint(ct_handle_t h, const double* L, const double* Wi, const double* Wj, const double* R,
+                    const double* theta, double* gWi, double* gWj, int64_t Dl, int64_t d, int64_t Dr,
+                    int64_t wL, int64_t wM, int64_t wR);
+
+/* ---- host-side input construction ---------------------------------------
+ * create_random_mps_c_compatible (src/tests/benchmarks/rng_c_compat.mojo:174-231):
+ * PCG32-seeded bond dimensions and float32 Box-Muller entries (widened to
+ * double).  Host buffers: dims[nsites+1]; data may be NULL to query the total
+ * number of values (returned in *total). */
+int ct_random_mps_c_compatible(int nsites, int64_t d, int64_t chi_max, uint64_t seed, int64_t* dims,
+                               double* data, int64_t* total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHEMTENSOR_HIP_H */

@@ -1,0 +1,324 @@
+"""GPU parity of the hot-path kernels against the FP64 oracle (through the C ABI).
+
+Sizes are chosen so the oracle finishes in seconds; odd sizes exercise the
+tile edges of the MFMA kernels.  FP64 results must agree to ~1e-12 relative;
+the FP32 and 3xBF16 forms carry the tolerances stated per test.
+"""
+import numpy as np
+import pytest
+import torch
+
+import chemtensor_mojo_amd as ct
+from chemtensor_mojo_amd import hamiltonians as ham
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(ctx, a, dtype=torch.float64):
+    return ct.create_dense_tensor_from_data(ctx, a, a.shape, dtype)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ------------------------------------------------------------- tensordot
+DOT_CASES = [
+    # (shapeA, shapeB, ndim_mult, axrange_A, axrange_B) - DT-DOT-001..008 (test_dense_tensor_dot.mojo:119-296)
+    ((3, 4), (4, 5), 1, False, True),
+    ((3, 4), (4, 5), 1, False, False),
+    ((2, 3, 4), (4, 5, 6), 1, False, True),
+    ((2, 3, 4, 5), (4, 5, 6, 7), 2, False, True),
+    ((4, 3), (4, 5), 1, True, True),
+    ((4, 3), (5, 4), 1, True, False),
+    ((3, 4), (5, 4), 1, False, False),
+    ((2, 3, 4, 5), (4, 5, 6, 7), 2, False, False),
+    ((2, 3, 4, 5), (6, 7, 4, 5), 2, False, False),
+    ((7,), (7,), 1, False, True),
+    # larger, tile edges
+    ((129, 70), (70, 131), 1, False, True),
+    ((300, 257), (300, 190), 1, True, True),
+    ((190, 33), (77, 33), 1, False, False),
+    ((64, 256), (128, 64), 1, True, False),
+]
+
+
+@pytest.mark.parametrize("case", DOT_CASES)
+def test_tensordot_fp64_vs_oracle(ctx, ora, case):
+    sA, sB, nm, aA, aB = case
+    rng = np.random.default_rng(sum(sA) * 7 + sum(sB))
+    A = rng.standard_normal(sA)
+    B = rng.standard_normal(sB)
+    exp = ora.tensordot(A, B, nm, aA, aB)
+    shapeC = list(exp.shape) if exp.size > 1 or len(sA) + len(sB) - 2 * nm > 0 else [1]
+    C = ct.create_dense_tensor(ctx, shapeC)
+    ct.dense_tensor_dot(C, dev(ctx, A), dev(ctx, B), ctx, nm, aA, aB)
+    assert rel(C.to_numpy().reshape(exp.shape), exp) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [(ct.PREC_FP32, 2e-6), (ct.PREC_BF16X3, 5e-5)])
+def test_tensordot_mixed_vs_oracle(ctx, ora, prec, tol):
+    rng = np.random.default_rng(5)
+    for sA, sB, nm, aA, aB in DOT_CASES[10:]:
+        A = rng.standard_normal(sA)
+        B = rng.standard_normal(sB)
+        exp = ora.tensordot(A, B, nm, aA, aB)
+        C = ct.create_dense_tensor(ctx, list(exp.shape), dtype=torch.float32)
+        ct.dense_tensor_dot(C, dev(ctx, A, torch.float32), dev(ctx, B, torch.float32), ctx, nm, aA, aB, prec=prec)
+        assert rel(C.to_numpy(), exp) < tol, (sA, sB)
+
+
+def test_tensordot_integer_kats(ctx):
+    """Known answers of test_c_replica_dot.mojo:34-163."""
+    def run(a, sa, b, sb, sc):
+        C = ct.create_dense_tensor(ctx, sc)
+        ct.dense_tensor_dot(C, dev(ctx, np.array(a, float).reshape(sa)), dev(ctx, np.array(b, float).reshape(sb)), ctx)
+        return C.to_numpy()
+    assert np.array_equal(run([1, 2, 3, 4, 5, 6], (2, 3), [7, 8, 9, 10, 11, 12], (3, 2), [2, 2]),
+                          [[58, 64], [139, 154]])
+    assert np.array_equal(run([1, 2, 3, 4, 5, 6], (2, 3), [10, 20, 30], (3,), [2]), [140, 320])
+    assert np.array_equal(run([1, 0, 0, 0, 2, 0, 0, 0, 3], (3, 3), list(range(1, 10)), (3, 3), [3, 3]),
+                          [[1, 2, 3], [8, 10, 12], [21, 24, 27]])
+    assert np.array_equal(run([1, 2, 3, 4], (2, 2), [5, 6, 7, 8], (2, 2), [2, 2]), [[19, 22], [43, 50]])
+    assert run([1, 2, 3, 4], (4,), [5, 6, 7, 8], (4,), [1])[0] == 70
+
+
+def test_tensordot_dim_mismatch_raises(ctx):
+    A = ct.create_dense_tensor(ctx, [3, 4])
+    B = ct.create_dense_tensor(ctx, [5, 6])
+    C = ct.create_dense_tensor(ctx, [3, 6])
+    with pytest.raises(ValueError):
+        ct.dense_tensor_dot(C, A, B, ctx)
+    B2 = ct.create_dense_tensor(ctx, [4, 6])
+    C_bad = ct.create_dense_tensor(ctx, [3, 5])
+    with pytest.raises(ValueError):
+        ct.dense_tensor_dot(C_bad, A, B2, ctx)
+
+
+def test_permute_vs_numpy(ctx):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((3, 4, 5, 6))
+    for perm in [(1, 0, 2, 3), (3, 2, 1, 0), (1, 3, 0, 2), (0, 2, 3, 1)]:
+        t = dev(ctx, a).transpose(list(perm), ctx)
+        assert np.array_equal(t.to_numpy(), np.transpose(a, perm))
+
+
+def test_blas1(ctx):
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(100003)
+    y = rng.standard_normal(100003)
+    tx, ty = dev(ctx, x), dev(ctx, y)
+    assert abs(tx.norm_sq(ctx) - np.dot(x, x)) < 1e-9 * np.dot(x, x)
+    assert abs(ct.dot_product(tx, ty, ctx) - np.dot(x, y)) < 1e-9 * np.abs(x).dot(np.abs(y))
+    ct.axpy_in_place(ty, 0.5, tx, ctx)
+    assert rel(ty.to_numpy(), y + 0.5 * x) < 1e-15
+    tx.scale_in_place(-2.0, ctx)
+    assert rel(tx.to_numpy(), -2.0 * x) < 1e-16
+
+
+# ------------------------------------------------------------ H_eff / envs
+def _rand_problem(rng, Dl, Dr, d, wL, wM, wR, Dlb=None, Drb=None):
+    Dlb = Dl if Dlb is None else Dlb
+    Drb = Dr if Drb is None else Drb
+    L = rng.standard_normal((wL, Dl, Dlb)) / np.sqrt(Dl)
+    R = rng.standard_normal((wR, Dr, Drb)) / np.sqrt(Dr)
+    Wi = rng.standard_normal((wL, d, d, wM))
+    Wj = rng.standard_normal((wM, d, d, wR))
+    th = rng.standard_normal((Dl, d, d, Dr))
+    return L, R, Wi, Wj, th
+
+
+HEFF_SHAPES = [(1, 2, 2, 1, 5, 1), (2, 4, 2, 5, 5, 5), (7, 9, 2, 5, 5, 5), (16, 16, 2, 8, 8, 8),
+               (33, 65, 2, 5, 5, 5), (64, 64, 2, 5, 5, 5), (130, 96, 2, 8, 8, 8), (40, 40, 3, 4, 4, 4)]
+
+
+@pytest.mark.parametrize("shape", HEFF_SHAPES)
+def test_heff_fp64_vs_oracle(ctx, ora, shape):
+    Dl, Dr, d, wL, wM, wR = shape
+    rng = np.random.default_rng(Dl * 131 + Dr)
+    L, R, Wi, Wj, th = _rand_problem(rng, Dl, Dr, d, wL, wM, wR)
+    exp = ora.heff_apply(L, Wi, Wj, R, th)
+    out = ct.apply_two_site_heff(dev(ctx, th), dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, Wi)),
+                                 ct.MPOSite(dev(ctx, Wj)), ctx)
+    assert rel(out.to_numpy(), exp) < 1e-13
+
+
+def test_heff_bra_dims_differ(ctx, ora):
+    rng = np.random.default_rng(11)
+    L, R, Wi, Wj, th = _rand_problem(rng, 12, 10, 2, 5, 5, 5, Dlb=7, Drb=9)
+    exp = ora.heff_apply(L, Wi, Wj, R, th)
+    out = ct.apply_two_site_heff(dev(ctx, th), dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, Wi)),
+                                 ct.MPOSite(dev(ctx, Wj)), ctx)
+    assert out.shape == [7, 2, 2, 9]
+    assert rel(out.to_numpy(), exp) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [(ct.PREC_FP32, 2e-6), (ct.PREC_BF16X3, 6e-5)])
+@pytest.mark.parametrize("shape", [(64, 64, 2, 5, 5, 5), (130, 96, 2, 8, 8, 8), (256, 256, 2, 8, 8, 8)])
+def test_heff_mixed_vs_oracle(ctx, ora, prec, tol, shape):
+    Dl, Dr, d, wL, wM, wR = shape
+    rng = np.random.default_rng(7 + Dl)
+    L, R, Wi, Wj, th = _rand_problem(rng, Dl, Dr, d, wL, wM, wR)
+    exp = ora.heff_apply(L, Wi, Wj, R, th)
+    for dt in (torch.float64, torch.float32):
+        out = ct.apply_two_site_heff(dev(ctx, th, dt), dev(ctx, L, dt), dev(ctx, R, dt), ct.MPOSite(dev(ctx, Wi, dt)),
+                                     ct.MPOSite(dev(ctx, Wj, dt)), ctx, prec=prec)
+        assert rel(out.to_numpy(), exp) < tol, dt
+
+
+ENV_SHAPES = [(1, 2, 2, 1, 5), (2, 2, 4, 5, 5), (7, 2, 13, 5, 5), (64, 2, 70, 8, 8), (33, 3, 20, 4, 4)]
+
+
+@pytest.mark.parametrize("shape", ENV_SHAPES)
+def test_env_updates_fp64_vs_oracle(ctx, ora, shape):
+    Dl, d, Dr, wL, wR = shape
+    rng = np.random.default_rng(Dl + 17 * Dr)
+    A = rng.standard_normal((Dl, d, Dr))
+    W = rng.standard_normal((wL, d, d, wR))
+    L = rng.standard_normal((wL, Dl, Dl))
+    R = rng.standard_normal((wR, Dr, Dr))
+    expL = ora.env_update_left(L, A, W)
+    expR = ora.env_update_right(R, A, W)
+    gotL = ct.update_left_environment(dev(ctx, L), ct.MPSSite(dev(ctx, A)), ct.MPOSite(dev(ctx, W)), ctx)
+    gotR = ct.update_right_environment(dev(ctx, R), ct.MPSSite(dev(ctx, A)), ct.MPOSite(dev(ctx, W)), ctx)
+    assert rel(gotL.to_numpy(), expL) < 1e-13
+    assert rel(gotR.to_numpy(), expR) < 1e-13
+
+
+@pytest.mark.parametrize("prec,tol", [(ct.PREC_FP32, 2e-6), (ct.PREC_BF16X3, 6e-5)])
+def test_env_updates_mixed(ctx, ora, prec, tol):
+    rng = np.random.default_rng(3)
+    Dl, d, Dr, wL, wR = 96, 2, 128, 8, 8
+    A = rng.standard_normal((Dl, d, Dr))
+    W = rng.standard_normal((wL, d, d, wR))
+    L = rng.standard_normal((wL, Dl, Dl))
+    R = rng.standard_normal((wR, Dr, Dr))
+    gotL = ct.update_left_environment(dev(ctx, L), ct.MPSSite(dev(ctx, A)), ct.MPOSite(dev(ctx, W)), ctx, prec=prec)
+    gotR = ct.update_right_environment(dev(ctx, R), ct.MPSSite(dev(ctx, A)), ct.MPOSite(dev(ctx, W)), ctx, prec=prec)
+    assert rel(gotL.to_numpy(), ora.env_update_left(L, A, W)) < tol
+    assert rel(gotR.to_numpy(), ora.env_update_right(R, A, W)) < tol
+
+
+# -------------------------------------------------------------- Lanczos
+def _sym_problem(rng, Dl, Dr, w=5):
+    """Hermitian H_eff: environments of a real MPO with symmetric operators, L/R symmetric per channel."""
+    d = 2
+    L = rng.standard_normal((w, Dl, Dl))
+    L = 0.5 * (L + L.transpose(0, 2, 1))
+    R = rng.standard_normal((w, Dr, Dr))
+    R = 0.5 * (R + R.transpose(0, 2, 1))
+    Wi = rng.standard_normal((w, d, d, w))
+    Wi = 0.5 * (Wi + Wi.transpose(0, 2, 1, 3))
+    Wj = rng.standard_normal((w, d, d, w))
+    Wj = 0.5 * (Wj + Wj.transpose(0, 2, 1, 3))
+    th = rng.standard_normal((Dl, d, d, Dr))
+    return L, R, Wi, Wj, th
+
+
+@pytest.mark.parametrize("dims", [(1, 2), (4, 4), (12, 20), (40, 33)])
+def test_lanczos_fp64_vs_oracle(ctx, ora, dims):
+    Dl, Dr = dims
+    rng = np.random.default_rng(Dl * 3 + Dr)
+    L, R, Wi, Wj, th = _sym_problem(rng, Dl, Dr)
+    E_o, v_o, nK_o, _, _ = ora.lanczos_two_site(th, L, R, Wi, Wj, max_iter=20, tol=1e-10)
+    E_g, v_g, info = ct.lanczos_two_site_optimize(dev(ctx, th), dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, Wi)),
+                                                  ct.MPOSite(dev(ctx, Wj)), ctx, 20, 1e-10, return_info=True)
+    assert info["num_iter"] == nK_o
+    assert abs(E_g - E_o) <= 1e-10 * max(1.0, abs(E_o))
+    vg = v_g.to_numpy().ravel()
+    vo = v_o.ravel()
+    assert min(rel(vg, vo), rel(-vg, vo)) < 1e-7
+
+
+def test_lanczos_breakdown_matches_oracle(ctx, ora):
+    """dim 4 < max_iter: the Krylov space is exhausted and beta falls below 100*dim*eps."""
+    rng = np.random.default_rng(9)
+    L, R, Wi, Wj, th = _sym_problem(rng, 1, 1)
+    E_o, _, nK_o, _, _ = ora.lanczos_two_site(th, L, R, Wi, Wj, max_iter=20, tol=1e-12)
+    E_g, _, info = ct.lanczos_two_site_optimize(dev(ctx, th), dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, Wi)),
+                                                ct.MPOSite(dev(ctx, Wj)), ctx, 20, 1e-12, return_info=True)
+    assert nK_o < 20 and info["num_iter"] == nK_o
+    assert abs(E_g - E_o) < 1e-10 * max(1.0, abs(E_o))
+
+
+def test_lanczos_zero_start_raises(ctx):
+    rng = np.random.default_rng(1)
+    L, R, Wi, Wj, th = _sym_problem(rng, 3, 3)
+    with pytest.raises(Exception, match="zero norm"):
+        ct.lanczos_two_site_optimize(dev(ctx, 0 * th), dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, Wi)),
+                                     ct.MPOSite(dev(ctx, Wj)), ctx, 20, 1e-10)
+
+
+@pytest.mark.parametrize("prec", [ct.PREC_FP32, ct.PREC_BF16X3])
+def test_lanczos_refined_energy(ctx, ora, prec):
+    """Mixed-precision Krylov matvecs + FP64 Rayleigh quotient: the refined energy
+    agrees with the FP64 Lanczos far better than the raw mixed-precision value."""
+    rng = np.random.default_rng(21)
+    L, R, Wi, Wj, th = _sym_problem(rng, 48, 48)
+    E_o, _, _, _, _ = ora.lanczos_two_site(th, L, R, Wi, Wj, max_iter=20, tol=1e-12)
+    E_r, _, info = ct.lanczos_two_site_optimize(dev(ctx, th), dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, Wi)),
+                                                ct.MPOSite(dev(ctx, Wj)), ctx, 20, 1e-12, prec=prec, refine=1,
+                                                return_info=True)
+    assert abs(E_r - E_o) / abs(E_o) < (1e-8 if prec == ct.PREC_FP32 else 1e-6)
+
+
+# ----------------------------------------------------------------- SVD
+def test_svd_kat(ctx):
+    """sigma = [3, 2] (test_svd_lapack.mojo:55-91)."""
+    A = np.array([[2.0, 0, 0], [0, 3.0, 0], [0, 0, 0], [0, 0, 0]])
+    U, S, Vt, chi, dw = ct.dense_tensor_svd_trunc(dev(ctx, A), ctx, chi_max=2, eps_trunc=1e-12)
+    assert chi == 2 and U.shape == [4, 2] and Vt.shape == [2, 3]
+    assert np.allclose(S.to_numpy(), [3.0, 2.0], atol=1e-14)
+
+
+@pytest.mark.parametrize("mn,chi_max,eps", [((16, 24), 10, 0.0), ((64, 48), 64, 1e-3), ((130, 70), 40, 1e-10),
+                                            ((256, 256), 200, 1e-8)])
+def test_svd_trunc_vs_oracle(ctx, ora, mn, chi_max, eps):
+    m, n = mn
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n)) @ np.diag(np.exp(-0.1 * np.arange(n)))
+    U_o, S_o, Vt_o, chi_o = ora.svd_trunc(A, chi_max, eps)
+    U, S, Vt, chi, dw = ct.dense_tensor_svd_trunc(dev(ctx, A), ctx, chi_max, eps)
+    assert chi == chi_o
+    assert rel(S.to_numpy(), S_o) < 1e-12
+    Ug, Vg = U.to_numpy(), Vt.to_numpy()
+    assert rel(Ug @ np.diag(S.to_numpy()) @ Vg, U_o @ np.diag(S_o) @ Vt_o) < 1e-11
+    assert np.allclose(Ug.T @ Ug, np.eye(chi), atol=1e-12)
+    tot = np.sum(A * A)
+    assert abs(dw - (tot - np.sum(S_o ** 2)) / tot) < 1e-12
+
+
+# ------------------------------------------------------------ full DMRG
+def _run_both(ctx, ora, mpo_t, mps_t, **kw):
+    p = ct.DMRGParams(**kw)
+    mpo = ct.mpo_from_tensors(ctx, mpo_t)
+    mps = ct.mps_from_tensors(ctx, mps_t)
+    E, out, st = ct.dmrg_two_site(ctx, mpo, mps, p, return_stats=True)
+    ref = ora.dmrg_two_site(mps_t, mpo_t, kw["num_sweeps"], kw["chi_max"], kw["eps_trunc"], kw["max_krylov_iter"],
+                            kw["krylov_tol"], kw["energy_tol"])
+    return E, out, st, ref
+
+
+def test_dmrg_config1_xxz_fm_vs_oracle(ctx, ora):
+    """L=16 reference XXZ (J=1, D=1: ferromagnet, E0 = -15), random MPS D=8, chi_max=32, 4 sweeps."""
+    mpo_t = ham.heisenberg_xxz_tensors(16, 1.0, 1.0, 0.0)
+    mps_t = ct.random_mps_c_compatible_tensors(16, 2, 8, 42)
+    E, out, st, ref = _run_both(ctx, ora, mpo_t, mps_t, num_sweeps=4, chi_max=32, eps_trunc=1e-10,
+                                max_krylov_iter=20, krylov_tol=1e-10, energy_tol=0.0)
+    assert abs(E - ref["E"]) <= 1e-10 * abs(ref["E"])
+    # the reference algorithm's fixed point with 20 Krylov vectors (200-rotation Jacobi cap)
+    assert abs(E + 14.98430786) < 1e-7
+    assert out.bond_dims == [s.shape[0] for s in ref["mps"]] + [1]
+
+
+def test_dmrg_heisenberg_afm_vs_oracle_and_ed(ctx, ora):
+    L = 10
+    mpo_t = ham.heisenberg_tensors(L, 1.0)
+    mps_t = ct.random_mps_c_compatible_tensors(L, 2, 8, 43)
+    E, out, st, ref = _run_both(ctx, ora, mpo_t, mps_t, num_sweeps=4, chi_max=32, eps_trunc=1e-12,
+                                max_krylov_iter=10, krylov_tol=1e-12, energy_tol=0.0)
+    H = ham.mpo_to_dense(mpo_t)
+    e0 = np.linalg.eigvalsh(H)[0]
+    assert abs(E - ref["E"]) <= 1e-10 * abs(ref["E"])
+    assert abs(E - e0) < 1e-8 * abs(e0)
