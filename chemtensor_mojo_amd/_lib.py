@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libdmrg_hip.so")
 
 PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
 F64, F32 = 0, 1
+SVD_GESVD, SVD_GRAM = 0, 1
 
 _i64 = ctypes.c_int64
 _i64p = ctypes.POINTER(ctypes.c_int64)
@@ -26,8 +27,8 @@ _dbl = ctypes.c_double
 
 class DMRGParamsC(ctypes.Structure):
     _fields_ = [("num_sweeps", _int), ("chi_max", _i64), ("eps_trunc", _dbl), ("max_krylov_iter", _int),
-                ("krylov_tol", _dbl), ("energy_tol", _dbl), ("jacobi_max_rot", _int), ("prec", _int),
-                ("refine", _int),
+                ("krylov_tol", _dbl), ("energy_tol", _dbl), ("jacobi_max_rot", _int), ("svd_mode", _int),
+                ("prec", _int), ("refine", _int),
                 ("env_prec", _int), ("verbose", _int)]
 
 
@@ -62,6 +63,7 @@ SIGNATURES = {
     "ct_lanczos_two_site": [_vp, _int, _vp, _vp, _vp, _vp, _vp] + [_i64] * 6
     + [_int, _dbl, _int, _int, _dp, _ip, _dp, _dp],
     "ct_svd_trunc": [_vp, _vp, _i64, _i64, _i64, _dbl, _vp, _vp, _vp, _i64p, _dp],
+    "ct_theta_split": [_vp, _int, _vp, _i64, _i64, _i64, _i64, _dbl, _int, _vp, _vp, _i64p, _dp],
     "ct_dmrg_create": [_vp, ctypes.POINTER(DMRGParamsC), _int, _i64, _i64p, ctypes.POINTER(_vp), _i64p,
                        ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "ct_dmrg_destroy": [_vp],
@@ -70,6 +72,7 @@ SIGNATURES = {
     "ct_dmrg_get_dims": [_vp, _i64p],
     "ct_dmrg_get_site": [_vp, _int, _vp],
     "ct_heff_adjoint": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp] + [_i64] * 6,
+    "ct_env_adjoint": [_vp, _vp, _vp, _vp, _vp] + [_i64] * 5,
     "ct_random_mps_c_compatible": [_int, _i64, _i64, ctypes.c_uint64, _i64p, _dp, _i64p],
 }
 _RESTYPES = {"ct_last_error": ctypes.c_char_p, "ct_version": ctypes.c_char_p}

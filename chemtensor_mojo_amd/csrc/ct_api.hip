@@ -50,13 +50,10 @@ int ct_handle_create(int device, void* stream, ct_handle_t* out) {
     CT_HIP(hipSetDevice(device));
     ct_handle_s* h = new ct_handle_s();
     h->device = device;
-    if (stream) {
-        h->stream = (hipStream_t)stream;
-        h->own_stream = false;
-    } else {
-        CT_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        h->own_stream = true;
-    }
+    // NULL is the default (null) stream, as everywhere in HIP: the library
+    // then orders with every other user of the default stream.
+    h->stream = (hipStream_t)stream;
+    h->own_stream = false;
     if (rocblas_create_handle(&h->blas) != rocblas_status_success) {
         delete h;
         ct::set_last_error("rocblas_create_handle failed");
@@ -89,13 +86,8 @@ int ct_handle_set_stream(ct_handle_t h, void* stream) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
     CT_HIP(hipStreamSynchronize(h->stream));
     if (h->own_stream) CT_HIP(hipStreamDestroy(h->stream));
-    if (stream) {
-        h->stream = (hipStream_t)stream;
-        h->own_stream = false;
-    } else {
-        CT_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-        h->own_stream = true;
-    }
+    h->stream = (hipStream_t)stream;  // NULL = the default stream
+    h->own_stream = false;
     rocblas_set_stream(h->blas, h->stream);
     return 0;
 }
@@ -265,6 +257,20 @@ int ct_svd_trunc(ct_handle_t h, const double* A, int64_t m, int64_t n, int64_t c
     return 0;
 }
 
+int ct_theta_split(ct_handle_t h, int svd_mode, const double* theta, int64_t Dl, int64_t d, int64_t Dr,
+                   int64_t chi_max, double eps, int left_to_right, double* A_i, double* A_j, int64_t* chi,
+                   double* discarded_weight) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(svd_mode == CT_SVD_GESVD || svd_mode == CT_SVD_GRAM, 2, "unknown svd_mode");
+    CT_ARG(Dl > 0 && d > 0 && Dr > 0, 4, "dimensions must be > 0");
+    CT_ARG(chi_max >= 1, 7, "chi_max must be >= 1");
+    ct::SvdResult r;
+    CT_CHECK(ct::theta_split(h, svd_mode, theta, Dl * d, d * Dr, chi_max, eps, left_to_right != 0, A_i, A_j, &r));
+    if (chi) *chi = r.chi;
+    if (discarded_weight) *discarded_weight = r.discarded;
+    return 0;
+}
+
 int ct_heff_adjoint(ct_handle_t h, const double* L, const double* Wi, const double* Wj, const double* R,
                     const double* theta, double* gWi, double* gWj, int64_t Dl, int64_t d, int64_t Dr, int64_t wL,
                     int64_t wM, int64_t wR) {
@@ -272,6 +278,13 @@ int ct_heff_adjoint(ct_handle_t h, const double* L, const double* Wi, const doub
     CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wM > 0 && wR > 0, 9, "dimensions must be > 0");
     ct::HeffDims hd{Dl, Dl, d, Dr, Dr, wL, wM, wR};
     return ct::heff_adjoint(h, L, Wi, Wj, R, theta, gWi, gWj, hd);
+}
+
+int ct_env_adjoint(ct_handle_t h, const double* L, const double* A, const double* R, double* gW, int64_t Dl,
+                   int64_t d, int64_t Dr, int64_t wL, int64_t wR) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wR > 0, 6, "dimensions must be > 0");
+    return ct::env_adjoint(h, L, A, R, gW, Dl, d, Dr, wL, wR);
 }
 
 }  // extern "C"

@@ -401,22 +401,21 @@ __global__ void adj_contract_kernel(const double* __restrict__ G, const double* 
     }
 }
 
-// G[wl, s1, s2 ; wr, s1o, s2o] = sum_{a', b} T1[wl][a'][s1][s2][b] * X[wr][a'][s1o][s2o][b]
-// One block per output element, reduction over (a', b); b is contiguous in
-// both operands.
+// G[(wl, e) ; (wr, eo)] = sum_{a', b} T1[wl][a'][e][b] * X[wr][a'][eo][b], e/eo < E
+// (E = d*d for the two-site network, d for one site).  One block per output
+// element, reduction over (a', b); b is contiguous in both operands.
 __global__ __launch_bounds__(256) void adj_gram_kernel(const double* __restrict__ T1, const double* __restrict__ X,
-                                                       int64_t Dlb, int64_t Dr, int d, int wR, double* __restrict__ G) {
-    const int NO = wR * d * d;
-    const int o = blockIdx.x;  // over wL*d*d*NO
+                                                       int64_t P, int64_t Q, int E, int wR, double* __restrict__ G) {
+    const int NO = wR * E;
+    const int o = blockIdx.x;  // over wL*E*NO
     const int k = o / NO, c = o - k * NO;
-    const int wl = k / (d * d), s12 = k % (d * d);
-    const int wr = c / (d * d), s12o = c % (d * d);
-    const int64_t d2 = d * d;
+    const int wl = k / E, e = k % E;
+    const int wr = c / E, eo = c % E;
     double s = 0.0;
-    const int64_t n = Dlb * Dr;
+    const int64_t n = P * Q;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const int64_t a = i / Dr, b = i - a * Dr;
-        s += T1[((wl * Dlb + a) * d2 + s12) * Dr + b] * X[(((int64_t)wr * Dlb + a) * d2 + s12o) * Dr + b];
+        const int64_t a = i / Q, b = i - a * Q;
+        s += T1[((wl * P + a) * E + e) * Q + b] * X[(((int64_t)wr * P + a) * E + eo) * Q + b];
     }
     s = block_sum256(s);
     if (threadIdx.x == 0) G[o] = s;
@@ -449,7 +448,7 @@ int heff_adjoint(Handle* h, const double* L, const double* Wi, const double* Wj,
         CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, theta, CT_F64, R, CT_F64, x, q));
     }
     hipLaunchKernelGGL(adj_gram_kernel, dim3((unsigned)nG), dim3(256), 0, h->stream, (const double*)t1,
-                       (const double*)x, D.Dlb, D.Dr, (int)D.d, (int)D.wR, (double*)g);
+                       (const double*)x, D.Dlb, D.Dr, (int)d2, (int)D.wR, (double*)g);
     CT_HIP(hipGetLastError());
     if (gWi) {
         const int n = (int)(D.wL * d2 * D.wM);
@@ -464,6 +463,45 @@ int heff_adjoint(Handle* h, const double* L, const double* Wi, const double* Wj,
         CT_HIP(hipGetLastError());
     }
     return 0;
+}
+
+// Single-site environment adjoint:
+//   gW[wl,s,s',wr] = sum L[wl,a,a'] A[a,s,b] A[a',s',b'] R[wr,b,b']
+// the derivative of <psi|H|psi> = L_i . A W A . R_{i+1} with respect to W.
+//   T1[wl][a'][s][b] = sum_a L[wl][a][a'] A[a][(s b)]        (GEMM batched over wl)
+//   Y [wr][a'][s'][b] = sum_b' A[(a' s')][b'] R[wr][b][b']   (GEMM batched over wr)
+//   gW = sum_{a',b} T1 Y                                       (adj_gram_kernel, E = d)
+int env_adjoint(Handle* h, const double* L, const double* A, const double* R, double* gW, int64_t Dl, int64_t d,
+                int64_t Dr, int64_t wL, int64_t wR) {
+    const int64_t nT1 = wL * Dl * d * Dr, nY = wR * Dl * d * Dr, nG = wL * d * d * wR;
+    void *t1, *y;
+    CT_CHECK(ws_get(h, WS_TMP, (size_t)nT1 * 8, &t1));
+    CT_CHECK(ws_get(h, WS_TMP2, (size_t)nY * 8, &y));
+    {
+        GemmDesc q;
+        q.M = (int)Dl; q.N = (int)(d * Dr); q.K = (int)Dl; q.batch = (int)wL;
+        q.a_kc = 0; q.sAm = 1; q.sAk = Dl; q.bsA = Dl * Dl;
+        q.b_kc = 0; q.sBk = d * Dr; q.sBn = 1; q.bsB = 0;
+        q.ldc = d * Dr; q.bsC = Dl * d * Dr;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, L, CT_F64, A, CT_F64, t1, q));
+    }
+    {
+        GemmDesc q;
+        q.M = (int)(Dl * d); q.N = (int)Dr; q.K = (int)Dr; q.batch = (int)wR;
+        q.a_kc = 1; q.sAm = Dr; q.sAk = 1; q.bsA = 0;
+        q.b_kc = 1; q.sBn = Dr; q.sBk = 1; q.bsB = Dr * Dr;
+        q.ldc = Dr; q.bsC = Dl * d * Dr;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, A, CT_F64, R, CT_F64, y, q));
+    }
+    void* g;
+    CT_CHECK(ws_get(h, WS_TMP3, (size_t)nG * 8, &g));
+    hipLaunchKernelGGL(adj_gram_kernel, dim3((unsigned)nG), dim3(256), 0, h->stream, (const double*)t1,
+                       (const double*)y, Dl, Dr, (int)d, (int)wR, (double*)g);
+    CT_HIP(hipGetLastError());
+    // G[wl][s][wr][s'] -> gW[wl][s][s'][wr]
+    const int64_t shape[4] = {wL, d, wR, d};
+    const int perm[4] = {0, 1, 3, 2};
+    return permute(h, CT_F64, g, 4, shape, perm, gW);
 }
 
 }  // namespace ct

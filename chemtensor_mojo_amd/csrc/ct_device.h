@@ -24,4 +24,21 @@ __device__ __forceinline__ double block_sum256(double v) {
     return red[0] + red[1] + red[2] + red[3];
 }
 
+// Tile coordinates of a GEMM workgroup.  Consecutive workgroup ids are
+// dispatched round-robin over the 8 XCDs of an MI355X, each with its own L2.
+// The id is first regrouped so that every XCD owns a contiguous range of
+// tiles, then walked in groups of GM tile rows (column-major inside a group),
+// so the A row-panels and B column-panels one XCD touches overlap and stay in
+// its L2.
+__device__ __forceinline__ void gemm_tile_coords(int bid, int nb, int tilesM, int tilesN, int& tm, int& tn) {
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    constexpr int GM = 8;
+    const int group = bid / (GM * tilesN);
+    const int first = group * GM;
+    const int gm = min(tilesM - first, GM);
+    const int in = bid - group * GM * tilesN;
+    tm = first + in % gm;
+    tn = in / gm;
+}
+
 }  // namespace ct

@@ -19,25 +19,6 @@
 
 namespace {
 
-// x[r][c] *= s[r]   (S absorbed into Vt, left-to-right: dmrg.mojo:206-230)
-__global__ void scale_rows_kernel(double* __restrict__ x, const double* __restrict__ s, int64_t rows, int64_t cols) {
-    const int64_t n = rows * cols;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] *= s[i / cols];
-}
-
-// x[r][c] *= s[c]   (S absorbed into U, right-to-left: dmrg.mojo:231-250)
-__global__ void scale_cols_kernel(double* __restrict__ x, const double* __restrict__ s, int64_t rows, int64_t cols) {
-    const int64_t n = rows * cols;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] *= s[i % cols];
-}
-
-inline int grid_of(int64_t n) {
-    int64_t g = (n + 255) / 256;
-    return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
-}
-
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -124,21 +105,15 @@ int local_update(ct_dmrg_s* s, int i, bool l2r, double* E_out) {
     CT_CHECK(ct::lanczos(h, env, &env64, s->theta, s->p.max_krylov_iter, s->p.krylov_tol, s->p.jacobi_max_rot,
                          s->p.refine, &lr));
     double t2 = now_s();
-    // split_two_site_theta_svd (dmrg.mojo:143-252): U -> A_i, Vt -> A_{i+1}
+    // split_two_site_theta_svd (dmrg.mojo:143-252): A_i = U (S) , A_{i+1} = (S) Vt
     ct::SvdResult sr;
-    CT_CHECK(ct::svd_trunc(h, s->theta, m, n, s->p.chi_max, s->p.eps_trunc, s->A[i], s->S, s->A[i + 1], &sr));
+    CT_CHECK(ct::theta_split(h, s->p.svd_mode, s->theta, m, n, s->p.chi_max, s->p.eps_trunc, l2r, s->A[i],
+                             s->A[i + 1], &sr));
     if (sr.chi <= 0) {
         ct::set_last_error("truncation kept no singular value");
         return CT_STATUS_SVD_FAILED;
     }
     const int64_t chi = sr.chi;
-    if (l2r)
-        hipLaunchKernelGGL(scale_rows_kernel, dim3(grid_of(chi * n)), dim3(256), 0, h->stream, s->A[i + 1],
-                           (const double*)s->S, chi, n);
-    else
-        hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_of(m * chi)), dim3(256), 0, h->stream, s->A[i],
-                           (const double*)s->S, m, chi);
-    CT_HIP(hipGetLastError());
     s->D[i + 1] = chi;
     CT_HIP(hipStreamSynchronize(h->stream));
     double t3 = now_s();
@@ -188,6 +163,7 @@ int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int6
     CT_ARG(params->prec >= 0 && params->prec <= 2, 2, "unknown precision");
     CT_ARG(params->env_prec >= 0 && params->env_prec <= 2, 2, "unknown environment precision");
     CT_ARG(params->jacobi_max_rot >= 0, 2, "jacobi_max_rot must be >= 0");
+    CT_ARG(params->svd_mode == CT_SVD_GESVD || params->svd_mode == CT_SVD_GRAM, 2, "unknown svd_mode");
     ct_dmrg_s* s = new ct_dmrg_s();
     s->h = h;
     s->p = *params;

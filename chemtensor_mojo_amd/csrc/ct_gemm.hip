@@ -3,7 +3,7 @@
 // One templated kernel serves the FP64 (v_mfma_f64_16x16x4_f64) and FP32
 // (v_mfma_f32_16x16x4_f32) paths: both MFMAs take one A and one B element per
 // lane (row/col = lane%16, k = lane/16) and return four accumulators per lane
-// (row = 4*(lane/16)+r, col = lane%16).  The 3xBF16 path (ct_gemm_bf16.hip)
+// in column lane%16 (rows: see Mfma<T>::row).  The 3xBF16 path (ct_gemm_bf16.hip)
 // uses v_mfma_f32_16x16x32_bf16 with a k-contiguous LDS layout.
 //
 // Block tile 128x128, K-tile 16, 256 threads = 4 waves in a 2x2 grid, each
@@ -14,6 +14,7 @@
 // transposes the reference performs with separate permute kernels,
 // dense_tensor.mojo:1249-1257, happen here inside the tile load).
 #include "ct_internal.h"
+#include "ct_device.h"
 
 namespace ct {
 namespace {
@@ -22,18 +23,24 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
 
+// row(lk, r): row of the 16x16 output tile held in accumulator r of a lane
+// with lane/16 == lk (the column is lane%16 for both).  The FP64 MFMA
+// interleaves rows across the four lane groups; the FP32 one gives each lane
+// group four consecutive rows.
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
     typedef v4d acc_t;
     static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
 };
 template <> struct Mfma<float> {
     typedef v4f acc_t;
     static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
 struct KArgs {
@@ -135,12 +142,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     __shared__ TC sB[BK][BN + PAD];
     typedef typename Mfma<TC>::acc_t acc_t;
 
-    // XCD-aware tile order: consecutive tile ids (which share operand rows)
-    // go to the same XCD, whose L2 then serves their common tiles.
-    int bid = blockIdx.x;
-    const int nb = gridDim.x;
-    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
-    const int tm = bid % p.tilesM, tn = bid / p.tilesM;
+    int tm, tn;
+    gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int z = blockIdx.z;
     const TA* A = (const TA*)p.A + (int64_t)z * p.bsA;
@@ -195,7 +198,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
             if (n >= p.N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm * 64 + i * 16 + lk * 4 + r;
+                const int m = m0 + wm * 64 + i * 16 + Mfma<TC>::row(lk, r);
                 if (m < p.M) {
                     TO* c = C + (int64_t)m * p.ldc + n;
                     double v = alpha * (double)acc[i][j][r];

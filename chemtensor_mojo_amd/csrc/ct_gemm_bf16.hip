@@ -15,6 +15,7 @@
 // Block tile 128x128x32, 256 threads = 4 waves (2x2), 64x64 per wave,
 // LDS double-buffered (2 x 40 KiB).
 #include "ct_internal.h"
+#include "ct_device.h"
 
 namespace ct {
 namespace {
@@ -94,7 +95,7 @@ struct NtArgs {
     const uint16_t *Ah, *Al, *Bh, *Bl;
     void* C;
     int M, N, K;
-    int tilesM;
+    int tilesM, tilesN;
     int vec;  // 16-byte loads legal (ld % 8 == 0, 16B aligned, K-tail handled separately)
     int64_t lda, ldb, bsA, bsB, ldc, bsC;
     double alpha, beta;
@@ -139,10 +140,8 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
     __shared__ __attribute__((aligned(16))) uint16_t sBh[2][BN][LDK];
     __shared__ __attribute__((aligned(16))) uint16_t sBl[2][BN][LDK];
 
-    int bid = blockIdx.x;
-    const int nb = gridDim.x;
-    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
-    const int tm = bid % p.tilesM, tn = bid / p.tilesM;
+    int tm, tn;
+    gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int z = blockIdx.z;
     const uint16_t* Ah = p.Ah + (int64_t)z * p.bsA;
@@ -263,8 +262,8 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
     auto al16 = [](const void* q) { return ((uintptr_t)q % 16) == 0; };
     p.vec = (A.ld % 8 == 0) && (B.ld % 8 == 0) && (A.bs % 8 == 0) && (B.bs % 8 == 0) && al16(A.hi) &&
             al16(A.lo) && al16(B.hi) && al16(B.lo);
-    const int tilesN = (N + BN - 1) / BN;
-    dim3 grid(p.tilesM * tilesN, 1, batch), block(256);
+    p.tilesN = (N + BN - 1) / BN;
+    dim3 grid(p.tilesM * p.tilesN, 1, batch), block(256);
     if (tC == CT_F64)
         hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, grid, block, 0, h->stream, p);
     else

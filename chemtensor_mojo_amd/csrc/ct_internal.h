@@ -203,6 +203,8 @@ int env_right(Handle* h, int prec, int dtype, const void* R, const void* A, cons
 int tensordot(Handle* h, int prec, int dtype, const void* A, int rankA, const int64_t* shapeA, const void* B,
               int rankB, const int64_t* shapeB, int ndim_mult, int axrange_A, int axrange_B, void* C, int rankC,
               const int64_t* shapeC);
+int env_adjoint(Handle* h, const double* L, const double* A, const double* R, double* gW, int64_t Dl, int64_t d,
+                int64_t Dr, int64_t wL, int64_t wR);
 int heff_adjoint(Handle* h, const double* L, const double* Wi, const double* Wj, const double* R,
                  const double* theta, double* gWi, double* gWj, const HeffDims& hd);
 
@@ -226,6 +228,12 @@ struct SvdResult {
     double norm_sq = 0.0;
 };
 // A: [m, n] row-major FP64 (device, preserved).  U: [m, chi] row-major, S[chi], Vt[chi, n].
+// split_two_site_theta_svd: A_i, A_j with S absorbed (into A_j when l2r).
+int theta_split(Handle* h, int mode, const double* theta, int64_t m, int64_t n, int64_t chi_max, double eps, bool l2r,
+                double* Ai, double* Aj, SvdResult* r);
+// Gram-matrix split writing the site tensors with S absorbed (see ct_svd.hip).
+int svd_split_gram(Handle* h, const double* theta, int64_t m, int64_t n, int64_t chi_max, double eps, bool l2r,
+                   double* Ai, double* Aj, SvdResult* r);
 int svd_trunc(Handle* h, const double* A, int64_t m, int64_t n, int64_t chi_max, double eps, double* U,
               double* S, double* Vt, SvdResult* r);
 

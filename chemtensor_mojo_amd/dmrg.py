@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import PREC_BF16X3, PREC_FP32, PREC_FP64, call
+from ._lib import PREC_BF16X3, PREC_FP32, PREC_FP64, SVD_GESVD, SVD_GRAM, call
 from .dense_tensor import DenseTensor, DeviceContext, default_prec
 from .state import MatrixProductOperator, MatrixProductState, MPOSite, MPSSite
 
@@ -31,6 +31,8 @@ class DMRGParams:
     jacobi_max_rot  rotation cap of the Jacobi eigensolve of T; 200 is the
                     reference's value (dmrg.mojo:510), which leaves 20x20 T
                     matrices unconverged late in a run
+    svd_mode        SVD_GESVD (full SVD, the reference's factorisation) or
+                    SVD_GRAM (eigendecomposition of theta theta^T / theta^T theta)
     """
     num_sweeps: int
     chi_max: int
@@ -44,6 +46,7 @@ class DMRGParams:
     refine: int = 0
     env_prec: int = PREC_FP64
     jacobi_max_rot: int = 200
+    svd_mode: int = SVD_GESVD
 
 
 @dataclass
@@ -149,21 +152,26 @@ def dense_tensor_svd_trunc(tensor: DenseTensor, ctx: DeviceContext, chi_max: int
 
 
 def split_two_site_theta_svd(theta: DenseTensor, chi_max: int, eps_trunc: float, ctx: DeviceContext,
-                             left_to_right: bool = True):
+                             left_to_right: bool = True, svd_mode: int = SVD_GESVD):
     """theta[Dl,d,d,Dr] -> (A_i[Dl,d,chi], A_{i+1}[chi,d,Dr], discarded_weight)
-    (dmrg.mojo:143-252): S goes right on L->R, left on R->L."""
+    (dmrg.mojo:143-252): S goes right on L->R, left on R->L.  FP64 on the GPU."""
+    if theta.dtype != torch.float64:
+        raise TypeError("split_two_site_theta_svd works in FP64")
     Dl, d, _, Dr = theta.shape
-    U, S, Vt, chi, dw = dense_tensor_svd_trunc(theta.reshape([Dl * d, d * Dr]), ctx, chi_max, eps_trunc)
-    if chi == 0:
+    k = min(Dl * d, d * Dr, chi_max)
+    dev = theta.storage.device
+    Ai = torch.empty(Dl * d * k, dtype=torch.float64, device=dev)
+    Aj = torch.empty(k * d * Dr, dtype=torch.float64, device=dev)
+    chi = ctypes.c_int64()
+    dw = ctypes.c_double()
+    call("ct_theta_split", ctx.handle, int(svd_mode), theta.ptr(), Dl, d, Dr, chi_max, float(eps_trunc),
+         int(bool(left_to_right)), ctypes.c_void_p(Ai.data_ptr()), ctypes.c_void_p(Aj.data_ptr()), ctypes.byref(chi),
+         ctypes.byref(dw))
+    c = chi.value
+    if c == 0:
         raise RuntimeError("truncation kept no singular value")
-    s = S.storage
-    if left_to_right:
-        A_j = (Vt.storage.view(chi, d * Dr) * s[:, None]).reshape(-1)
-        A_i = U.storage
-    else:
-        A_i = (U.storage.view(Dl * d, chi) * s[None, :]).reshape(-1)
-        A_j = Vt.storage
-    return MPSSite(DenseTensor(A_i.contiguous(), [Dl, d, chi])), MPSSite(DenseTensor(A_j.contiguous(), [chi, d, Dr])), dw
+    return (MPSSite(DenseTensor(Ai[: Dl * d * c], [Dl, d, c])), MPSSite(DenseTensor(Aj[: c * d * Dr], [c, d, Dr])),
+            dw.value)
 
 
 class DMRGRun:
@@ -185,8 +193,8 @@ class DMRGRun:
         self._mpo = [s.tensor.storage.to(torch.float64) for s in mpo.sites]
         self._mps = [s.tensor.storage.to(torch.float64) for s in mps.sites]
         p = _lib.DMRGParamsC(params.num_sweeps, params.chi_max, params.eps_trunc, params.max_krylov_iter,
-                             params.krylov_tol, params.energy_tol, params.jacobi_max_rot, params.prec, params.refine,
-                             params.env_prec, int(params.verbose))
+                             params.krylov_tol, params.energy_tol, params.jacobi_max_rot, params.svd_mode, params.prec,
+                             params.refine, params.env_prec, int(params.verbose))
         wd = _lib.i64_array(mpo.bond_dims)
         md = _lib.i64_array(mps.bond_dims)
         wp = (ctypes.c_void_p * N)(*[t.data_ptr() for t in self._mpo])

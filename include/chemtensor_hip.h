@@ -37,6 +37,9 @@ enum {
 };
 /* element type of buffers */
 enum { CT_F64 = 0, CT_F32 = 1 };
+/* two-site split: full SVD (rocSOLVER gesvd), or eigendecomposition of the
+ * Gram matrix theta theta^T / theta^T theta (rocSOLVER syevd + 2 GEMMs) */
+enum { CT_SVD_GESVD = 0, CT_SVD_GRAM = 1 };
 
 enum {
     CT_STATUS_OK = 0,
@@ -55,7 +58,7 @@ typedef struct ct_dmrg_s* ct_dmrg_t;
 
 /* ---- handle ------------------------------------------------------------
  * Replaces the reference's DeviceContext for this path (src/algorithms/dmrg.mojo:378).
- * stream == NULL creates a library-owned stream (default priority). */
+ * All work is enqueued on `stream`; NULL is the default (null) stream. */
 int ct_handle_create(int device, void* stream, ct_handle_t* out);
 int ct_handle_destroy(ct_handle_t h);
 int ct_handle_set_stream(ct_handle_t h, void* stream);
@@ -152,6 +155,15 @@ int ct_lanczos_two_site(ct_handle_t h, int prec, const double* L, const double* 
 int ct_svd_trunc(ct_handle_t h, const double* A, int64_t m, int64_t n, int64_t chi_max, double eps,
                  double* U, double* S, double* Vt, int64_t* chi, double* discarded_weight);
 
+/* split_two_site_theta_svd (src/algorithms/dmrg.mojo:143-252): theta
+ * [Dl,d,d,Dr] (FP64, preserved) -> A_i [Dl,d,chi], A_j [chi,d,Dr] with S
+ * absorbed into A_j (left_to_right) or into A_i (right-to-left).  A_i / A_j
+ * must hold min(Dl*d, d*Dr, chi_max) bond columns / rows.  svd_mode selects
+ * CT_SVD_GESVD or CT_SVD_GRAM.  Synchronises the stream. */
+int ct_theta_split(ct_handle_t h, int svd_mode, const double* theta, int64_t Dl, int64_t d, int64_t Dr,
+                   int64_t chi_max, double eps, int left_to_right, double* A_i, double* A_j, int64_t* chi,
+                   double* discarded_weight);
+
 /* ---- sweep driver ------------------------------------------------------
  * dmrg_two_site (src/algorithms/dmrg.mojo:377-483) with DMRGWorkspace
  * (:39-90) and dmrg_local_update_two_site (:759-844).  The MPS and MPO are
@@ -166,6 +178,7 @@ typedef struct {
     double krylov_tol;     /* DMRGParams.krylov_tol (Jacobi tolerance) */
     double energy_tol;     /* DMRGParams.energy_tol */
     int jacobi_max_rot;    /* Jacobi rotation cap of the local eigensolve (reference: 200, dmrg.mojo:510) */
+    int svd_mode;          /* CT_SVD_GESVD (reference factorisation) or CT_SVD_GRAM */
     int prec;              /* matvec precision CT_PREC_* */
     int refine;            /* 0: reference energies; 1: FP64 Rayleigh quotient; 2: + residual step */
     int env_prec;          /* precision of the environment updates (CT_PREC_*) */
@@ -213,6 +226,12 @@ int ct_dmrg_get_site(ct_dmrg_t s, int i, double* dst);
 int ct_heff_adjoint(ct_handle_t h, const double* L, const double* Wi, const double* Wj, const double* R,
                     const double* theta, double* gWi, double* gWj, int64_t Dl, int64_t d, int64_t Dr,
                     int64_t wL, int64_t wM, int64_t wR);
+
+/* Single-site form: dE/dW of E = L . A W A . R for site tensor A [Dl,d,Dr],
+ * L [wL,Dl,Dl], R [wR,Dr,Dr]:
+ *   gW[wl,s,s',wr] = sum L[wl,a,a'] A[a,s,b] A[a',s',b'] R[wr,b,b']   (FP64) */
+int ct_env_adjoint(ct_handle_t h, const double* L, const double* A, const double* R, double* gW, int64_t Dl,
+                   int64_t d, int64_t Dr, int64_t wL, int64_t wR);
 
 /* ---- host-side input construction ---------------------------------------
  * create_random_mps_c_compatible (src/tests/benchmarks/rng_c_compat.mojo:174-231):
