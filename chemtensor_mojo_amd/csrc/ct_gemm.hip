@@ -83,8 +83,9 @@ __device__ __forceinline__ void load4<float, double>(const float* p, double* o) 
 
 // Tile loader.  KC == false: operand X[k][x] (x contiguous, row stride sk);
 // KC == true: operand X[x][k] (k contiguous, row stride sx).  Fills 8
-// register values per thread for a BK x 128 tile at (k0, x0).
-template <typename TS, typename TC, bool KC>
+// register values per thread for a BK x 128 tile at (k0, x0).  FULL: the tile
+// is entirely inside the operand and 16-byte loads are legal (no checks).
+template <typename TS, typename TC, bool KC, bool FULL>
 __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, int X0, int XN, int k0, int K,
                                           bool vec, TC* r) {
     const int tid = threadIdx.x;
@@ -95,7 +96,7 @@ __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, i
             const int k = k0 + row + 8 * h;
             const int x = X0 + 4 * c;
             TC* o = r + 4 * h;
-            if (vec && k < K && x + 3 < XN) {
+            if (FULL || (vec && k < K && x + 3 < XN)) {
                 load4<TS, TC>(X + (int64_t)k * sk + x, o);
             } else {
 #pragma unroll
@@ -106,7 +107,7 @@ __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, i
     } else {
         const int x = X0 + (tid & 127), hk = tid >> 7;  // row x, k = k0+8hk .. +7
         const int kb = k0 + 8 * hk;
-        if (vec && x < XN && kb + 7 < K) {
+        if (FULL || (vec && x < XN && kb + 7 < K)) {
             const TS* p = X + (int64_t)x * sx + kb;
             load4<TS, TC>(p, r);
             load4<TS, TC>(p + 4, r + 4);
@@ -134,12 +135,15 @@ __device__ __forceinline__ void store_tile(TC (*S)[LD], const TC* r) {
     }
 }
 
-template <typename TC, typename TA, typename TB, typename TO, bool AKC, bool BKC>
+// LDS is double-buffered: the tile for k0+BK is loaded into registers before
+// the MFMAs on k0 and stored to the other buffer after them, so each K step
+// needs a single barrier.
+template <typename TC, typename TA, typename TB, typename TO, bool AKC, bool BKC, bool FULL>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     if (p.active && *p.active == 0) return;
     constexpr int PAD = 4;
-    __shared__ TC sA[BK][BM + PAD];
-    __shared__ TC sB[BK][BN + PAD];
+    __shared__ TC sA[2][BK][BM + PAD];
+    __shared__ TC sB[2][BK][BN + PAD];
     typedef typename Mfma<TC>::acc_t acc_t;
 
     int tm, tn;
@@ -163,30 +167,37 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     TC ra[8], rb[8];
     const int64_t sAx = AKC ? p.sAm : 0, sAk = AKC ? 0 : p.sAk;
     const int64_t sBx = BKC ? p.sBn : 0, sBk = BKC ? 0 : p.sBk;
-    load_tile<TA, TC, AKC>(A, sAx, sAk, m0, p.M, 0, p.K, p.vecA, ra);
-    load_tile<TB, TC, BKC>(B, sBx, sBk, n0, p.N, 0, p.K, p.vecB, rb);
+    load_tile<TA, TC, AKC, FULL>(A, sAx, sAk, m0, p.M, 0, p.K, p.vecA, ra);
+    load_tile<TB, TC, BKC, FULL>(B, sBx, sBk, n0, p.N, 0, p.K, p.vecB, rb);
+    store_tile<TC, AKC, BM + PAD>(sA[0], ra);
+    store_tile<TC, BKC, BN + PAD>(sB[0], rb);
+    __syncthreads();
 
+    int cur = 0;
     for (int k0 = 0; k0 < p.K; k0 += BK) {
-        __syncthreads();
-        store_tile<TC, AKC, BM + PAD>(sA, ra);
-        store_tile<TC, BKC, BN + PAD>(sB, rb);
-        __syncthreads();
-        if (k0 + BK < p.K) {
-            load_tile<TA, TC, AKC>(A, sAx, sAk, m0, p.M, k0 + BK, p.K, p.vecA, ra);
-            load_tile<TB, TC, BKC>(B, sBx, sBk, n0, p.N, k0 + BK, p.K, p.vecB, rb);
+        const bool more = k0 + BK < p.K;
+        if (more) {
+            load_tile<TA, TC, AKC, FULL>(A, sAx, sAk, m0, p.M, k0 + BK, p.K, p.vecA, ra);
+            load_tile<TB, TC, BKC, FULL>(B, sBx, sBk, n0, p.N, k0 + BK, p.K, p.vecB, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
             TC a[4], b[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = sA[kk * 4 + lk][wm * 64 + i * 16 + li];
+            for (int i = 0; i < 4; ++i) a[i] = sA[cur][kk * 4 + lk][wm * 64 + i * 16 + li];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = sB[kk * 4 + lk][wn * 64 + j * 16 + li];
+            for (int j = 0; j < 4; ++j) b[j] = sB[cur][kk * 4 + lk][wn * 64 + j * 16 + li];
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<TC>::op(a[i], b[j], acc[i][j]);
         }
+        if (more) {
+            store_tile<TC, AKC, BM + PAD>(sA[cur ^ 1], ra);
+            store_tile<TC, BKC, BN + PAD>(sB[cur ^ 1], rb);
+        }
+        __syncthreads();
+        cur ^= 1;
     }
 
     const double alpha = p.alpha, beta = p.beta;
@@ -195,11 +206,11 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int n = n0 + wn * 64 + j * 16 + li;
-            if (n >= p.N) continue;
+            if (!FULL && n >= p.N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int m = m0 + wm * 64 + i * 16 + Mfma<TC>::row(lk, r);
-                if (m < p.M) {
+                if (FULL || m < p.M) {
                     TO* c = C + (int64_t)m * p.ldc + n;
                     double v = alpha * (double)acc[i][j][r];
                     if (beta != 0.0) v += beta * (double)(*c);
@@ -230,14 +241,21 @@ int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDes
     p.vecA = akc ? aligned16<TA>(A, g.sAm, g.bsA) : aligned16<TA>(A, g.sAk, g.bsA);
     p.vecB = bkc ? aligned16<TB>(B, g.sBn, g.bsB) : aligned16<TB>(B, g.sBk, g.bsB);
     dim3 grid(p.tilesM * p.tilesN, 1, g.batch), block(256);
-    if (!akc && !bkc)
-        hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, false, false>), grid, block, 0, h->stream, p);
-    else if (akc && !bkc)
-        hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, true, false>), grid, block, 0, h->stream, p);
-    else if (!akc && bkc)
-        hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, false, true>), grid, block, 0, h->stream, p);
-    else
-        hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, true, true>), grid, block, 0, h->stream, p);
+    const bool full = p.vecA && p.vecB && (g.M % BM == 0) && (g.N % BN == 0) && (g.K % BK == 0);
+#define CT_LAUNCH(AK, BKc, F) \
+    hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, AK, BKc, F>), grid, block, 0, h->stream, p)
+    if (full) {
+        if (!akc && !bkc) CT_LAUNCH(false, false, true);
+        else if (akc && !bkc) CT_LAUNCH(true, false, true);
+        else if (!akc && bkc) CT_LAUNCH(false, true, true);
+        else CT_LAUNCH(true, true, true);
+    } else {
+        if (!akc && !bkc) CT_LAUNCH(false, false, false);
+        else if (akc && !bkc) CT_LAUNCH(true, false, false);
+        else if (!akc && bkc) CT_LAUNCH(false, true, false);
+        else CT_LAUNCH(true, true, false);
+    }
+#undef CT_LAUNCH
     CT_HIP(hipGetLastError());
     return 0;
 }
@@ -284,7 +302,7 @@ int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int 
             set_last_error("FP64 contraction requires FP64 buffers");
             return CT_STATUS_NOT_SUPPORTED;
         }
-        return dispatch_io<double>(h, tA, A, tB, B, tC, C, g);
+        return launch_typed<double, double, double, double>(h, A, B, C, g);
     }
     if (prec == CT_PREC_FP32) return dispatch_io<float>(h, tA, A, tB, B, tC, C, g);
     if (prec == CT_PREC_BF16X3) return gemm_bf16x3(h, tA, A, tB, B, tC, C, g);

@@ -12,7 +12,10 @@ namespace {
 // ------------------------------------------------------------------ mixing
 // Y[co, p, eo, q] = sum_{ci, ei} coef[(ci*EIN + ei)*NOUT + co*EOUT + eo] * X[ci, p, ei, q]
 // One thread per pixel (p, q); q is the contiguous index on both sides, so
-// every load and store of a wavefront is 64 consecutive elements.
+// every load and store of a wavefront is 64 consecutive elements.  All K
+// inputs of a pixel are loaded before any arithmetic (K loads in flight per
+// thread), and the coefficients are wave-uniform, so they are read through
+// the scalar cache straight into SGPR operands of the FMAs.
 constexpr int MIX_KMAX = 64;
 constexpr int MIX_NMAX = 64;
 
@@ -31,39 +34,56 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
     return (uint16_t)(u >> 16);
 }
 
-// OUT: 0 = double, 1 = float, 2 = split BF16 hi/lo planes (uint16)
-template <typename TX, typename TACC, int OUT>
+// OUT: 0 = double, 1 = float, 2 = split BF16 hi/lo planes (uint16).
+// KM: compile-time size (16, 32 or 64) of the zero-padded KM x KM coefficient
+// block `coef` (row k = input, column n = output).  Inputs k >= K re-read
+// input K-1 and meet zero coefficients, so the unrolled loops carry no
+// per-element guards; every coefficient address is a compile-time offset
+// from a kernel argument, so the FMAs take them from SGPRs loaded through
+// the scalar cache.
+template <typename TX, typename TACC, int OUT, int KM>
 __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void* __restrict__ Yv,
-                                                  const double* __restrict__ coef, MixArgs a) {
+                                                  const TACC* __restrict__ coef, MixArgs a) {
     if (a.active && *a.active == 0) return;
-    __shared__ TACC sc[MIX_KMAX * MIX_NMAX];
+    __shared__ int64_t in_off[KM];
+    __shared__ int64_t out_off[KM];
     const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
-    for (int i = threadIdx.x; i < K * NO; i += blockDim.x) sc[i] = (TACC)coef[i];
+    for (int k = threadIdx.x; k < KM; k += blockDim.x) {
+        const int kk = k < K ? k : K - 1;
+        const int ci = kk / a.EIN, ei = kk - ci * a.EIN;
+        in_off[k] = ci * a.sXc + ei * a.sXe;
+        const int nn = k < NO ? k : NO - 1;
+        const int co = nn / a.EOUT, eo = nn - co * a.EOUT;
+        out_off[k] = co * a.sYc + eo * a.sYe;
+    }
     __syncthreads();
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= a.Q) return;
     for (int64_t p = blockIdx.y; p < a.P; p += gridDim.y) {
-        TACC x[MIX_KMAX];
+        // Keeps the coefficient loads inside the loop: hoisting all KM*KM of
+        // them out of it would spill.
+        asm volatile("" ::: "memory");
+        const TX* xp = X + p * a.sXp + q;
+        TACC x[KM];
 #pragma unroll
-        for (int k = 0; k < MIX_KMAX; ++k) {
-            if (k < K) {
-                const int ci = k / a.EIN, ei = k - ci * a.EIN;
-                x[k] = (TACC)X[ci * a.sXc + p * a.sXp + ei * a.sXe + q];
-            }
-        }
-        for (int n = 0; n < NO; ++n) {
-            TACC acc = 0;
+        for (int k = 0; k < KM; ++k) x[k] = (TACC)xp[in_off[k]];
+        TACC acc[KM];
 #pragma unroll
-            for (int k = 0; k < MIX_KMAX; ++k)
-                if (k < K) acc += sc[k * NO + n] * x[k];
-            const int co = n / a.EOUT, eo = n - co * a.EOUT;
-            const int64_t o = co * a.sYc + p * a.sYp + eo * a.sYe + q;
+        for (int n = 0; n < KM; ++n) acc[n] = 0;
+#pragma unroll
+        for (int k = 0; k < KM; ++k)
+#pragma unroll
+            for (int n = 0; n < KM; ++n) acc[n] += coef[k * KM + n] * x[k];
+#pragma unroll
+        for (int n = 0; n < KM; ++n) {
+            if (n >= NO) break;
+            const int64_t o = out_off[n] + p * a.sYp + q;
             if (OUT == 0) {
-                ((double*)Yv)[o] = (double)acc;
+                ((double*)Yv)[o] = (double)acc[n];
             } else if (OUT == 1) {
-                ((float*)Yv)[o] = (float)acc;
+                ((float*)Yv)[o] = (float)acc[n];
             } else {
-                const float f = (float)acc;
+                const float f = (float)acc[n];
                 const uint16_t hb = f2bf(f);
                 const float hf = __uint_as_float(((uint32_t)hb) << 16);
                 ((uint16_t*)Yv)[o] = hb;
@@ -71,6 +91,16 @@ __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void
             }
         }
     }
+}
+
+// Zero-padded KM x KM copy of the [K][NO] FP64 coefficients in the
+// accumulation type.
+template <typename TACC>
+__global__ void pad_coef_kernel(const double* __restrict__ in, TACC* __restrict__ out, int K, int NO, int KM) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= KM * KM) return;
+    const int k = i / KM, n = i - k * KM;
+    out[i] = (k < K && n < NO) ? (TACC)in[k * NO + n] : (TACC)0;
 }
 
 // ------------------------------------------------------------------ permute
@@ -162,30 +192,63 @@ static inline int grid_for(int64_t n) {
     return (int)g;
 }
 
-int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef, const MixDesc& m) {
+namespace {
+
+int mix_args(const MixDesc& m, int64_t lo_off, MixArgs* a) {
     const int K = m.CIN * m.EIN, NO = m.COUT * m.EOUT;
     if (K > MIX_KMAX || NO > MIX_NMAX) {
         set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
         return CT_STATUS_NOT_SUPPORTED;
     }
+    a->CIN = m.CIN; a->EIN = m.EIN; a->COUT = m.COUT; a->EOUT = m.EOUT;
+    a->P = m.P; a->Q = m.Q;
+    a->sXc = m.sXc; a->sXp = m.sXp; a->sXe = m.sXe;
+    a->sYc = m.sYc; a->sYp = m.sYp; a->sYe = m.sYe;
+    a->lo_off = lo_off;
+    a->active = m.active;
+    return 0;
+}
+
+static int mix_km(const MixArgs& a) {
+    const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
+    return (K <= 16 && NO <= 16) ? 16 : (K <= 32 && NO <= 32) ? 32 : 64;
+}
+
+// Padded coefficient block (workspace) in the accumulation type, then the
+// mixing launch.
+template <typename TX, typename TACC, int OUT>
+int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixArgs& a) {
+    const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT, KM = mix_km(a);
+    void* w;
+    CT_CHECK(ws_get(h, WS_MIX, (size_t)KM * KM * sizeof(TACC), &w));
+    TACC* c = (TACC*)w;
+    hipLaunchKernelGGL(pad_coef_kernel<TACC>, dim3((KM * KM + 255) / 256), dim3(256), 0, h->stream, d_coef, c, K,
+                       NO, KM);
+    dim3 grid((unsigned)((a.Q + 255) / 256), (unsigned)(a.P < 65535 ? a.P : 65535)), block(256);
+    if (KM == 16)
+        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16>), grid, block, 0, h->stream, X, Y, c, a);
+    else if (KM == 32)
+        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32>), grid, block, 0, h->stream, X, Y, c, a);
+    else
+        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 64>), grid, block, 0, h->stream, X, Y, c, a);
+    return 0;
+}
+
+}  // namespace
+
+int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef, const MixDesc& m) {
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
-    a.CIN = m.CIN; a.EIN = m.EIN; a.COUT = m.COUT; a.EOUT = m.EOUT;
-    a.P = m.P; a.Q = m.Q;
-    a.sXc = m.sXc; a.sXp = m.sXp; a.sXe = m.sXe;
-    a.sYc = m.sYc; a.sYp = m.sYp; a.sYe = m.sYe;
-    a.lo_off = 0;
-    a.active = m.active;
-    dim3 grid((unsigned)((m.Q + 255) / 256), (unsigned)(m.P < 65535 ? m.P : 65535)), block(256);
-    if (tX == CT_F64 && tY == CT_F64)
-        hipLaunchKernelGGL((mix_kernel<double, double, 0>), grid, block, 0, h->stream, (const double*)X, Y, d_coef, a);
-    else if (tX == CT_F32 && tY == CT_F32)
-        hipLaunchKernelGGL((mix_kernel<float, float, 1>), grid, block, 0, h->stream, (const float*)X, Y, d_coef, a);
-    else if (tX == CT_F32 && tY == CT_F64)
-        hipLaunchKernelGGL((mix_kernel<float, float, 0>), grid, block, 0, h->stream, (const float*)X, Y, d_coef, a);
-    else if (tX == CT_F64 && tY == CT_F32)
-        hipLaunchKernelGGL((mix_kernel<double, double, 1>), grid, block, 0, h->stream, (const double*)X, Y, d_coef, a);
-    else {
+    CT_CHECK(mix_args(m, 0, &a));
+    if (tX == CT_F64 && tY == CT_F64) {
+        CT_CHECK((launch_mix<double, double, 0>(h, (const double*)X, Y, d_coef, a)));
+    } else if (tX == CT_F64 && tY == CT_F32) {
+        CT_CHECK((launch_mix<double, double, 1>(h, (const double*)X, Y, d_coef, a)));
+    } else if (tX == CT_F32 && tY == CT_F32) {
+        CT_CHECK((launch_mix<float, float, 1>(h, (const float*)X, Y, d_coef, a)));
+    } else if (tX == CT_F32 && tY == CT_F64) {
+        CT_CHECK((launch_mix<float, float, 0>(h, (const float*)X, Y, d_coef, a)));
+    } else {
         set_last_error("mix: unsupported types");
         return CT_STATUS_NOT_SUPPORTED;
     }
@@ -196,21 +259,10 @@ int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef,
 // Mixing whose output goes straight into BF16 hi/lo planes (lo plane at
 // Y + lo_off elements); X is FP32.
 int mix_split(Handle* h, const float* X, uint16_t* Y, int64_t lo_off, const double* d_coef, const MixDesc& m) {
-    const int K = m.CIN * m.EIN, NO = m.COUT * m.EOUT;
-    if (K > MIX_KMAX || NO > MIX_NMAX) {
-        set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
-        return CT_STATUS_NOT_SUPPORTED;
-    }
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
-    a.CIN = m.CIN; a.EIN = m.EIN; a.COUT = m.COUT; a.EOUT = m.EOUT;
-    a.P = m.P; a.Q = m.Q;
-    a.sXc = m.sXc; a.sXp = m.sXp; a.sXe = m.sXe;
-    a.sYc = m.sYc; a.sYp = m.sYp; a.sYe = m.sYe;
-    a.lo_off = lo_off;
-    a.active = m.active;
-    dim3 grid((unsigned)((m.Q + 255) / 256), (unsigned)(m.P < 65535 ? m.P : 65535)), block(256);
-    hipLaunchKernelGGL((mix_kernel<float, float, 2>), grid, block, 0, h->stream, X, (void*)Y, d_coef, a);
+    CT_CHECK(mix_args(m, lo_off, &a));
+    CT_CHECK((launch_mix<float, float, 2>(h, X, (void*)Y, d_coef, a)));
     CT_HIP(hipGetLastError());
     return 0;
 }

@@ -328,7 +328,14 @@ def _run_both(ctx, ora, mpo_t, mps_t, **kw):
 
 def test_dmrg_config1_xxz_fm_vs_oracle(ctx, ora):
     """L=16 reference XXZ (J=1, D=1: ferromagnet, E0 = -15), PCG32 MPS D=8 (seed 42), chi_max=32, 4 sweeps,
-    20 Krylov vectors, with the local eigensolve converged (Jacobi rotation cap lifted)."""
+    20 Krylov vectors, with the local eigensolve converged (Jacobi rotation cap lifted).
+
+    The Hamiltonian is SU(2)-symmetric, so singular values come in degenerate
+    multiplets; where chi_max=32 cuts through one during the first sweeps the
+    kept subspace depends on the (arbitrary) basis an SVD picks inside the
+    multiplet, so the intermediate sweep energies of two correct
+    implementations differ at the 1e-4 level.  The converged energy is fixed:
+    it must match the oracle to 1e-10 relative and the exact -15."""
     mpo_t = ham.heisenberg_xxz_tensors(16, 1.0, 1.0, 0.0)
     mps_t = ct.random_mps_c_compatible_tensors(16, 2, 8, 42)
     p = ct.DMRGParams(num_sweeps=4, chi_max=32, eps_trunc=1e-10, max_krylov_iter=20, krylov_tol=1e-10,
@@ -338,16 +345,18 @@ def test_dmrg_config1_xxz_fm_vs_oracle(ctx, ora):
     ref = ora.dmrg_two_site(mps_t, mpo_t, 4, 32, 1e-10, 20, 1e-10, 0.0, jacobi_max_rot=20000)
     assert abs(E - ref["E"]) <= 1e-10 * abs(ref["E"])
     assert abs(E + 15.0) < 1e-9
-    assert np.allclose(st.sweep_energies, ref["sweep_E"], rtol=1e-10, atol=0)
+    se = np.asarray(st.sweep_energies)
+    assert np.all(np.diff(se) <= 1e-12) and se[-1] == E
+    assert np.allclose(se, ref["sweep_E"], rtol=1e-3, atol=0)
 
 
 def test_dmrg_config1_reference_jacobi_cap_stalls(ctx, ora):
     """With the reference's 200-rotation Jacobi cap (dmrg.mojo:510) the 20-vector
-    Lanczos leaves the late 20x20 tridiagonal problems unconverged and both the
-    oracle and the GPU path stall above E0 = -15.  Where exactly they stall is
-    set by rounding-level differences amplified by the unconverged solves, so
-    the two stalled energies are compared only coarsely; the first sweep, before
-    the stall, agrees closely."""
+    Lanczos leaves the 20x20 tridiagonal problems unconverged and both the
+    oracle and the GPU path stall above E0 = -15.  The unconverged solves pick
+    Ritz vectors inside the SU(2) multiplets from rounding-level differences,
+    so the two trajectories separate from the first sweep on; only the stall
+    itself is compared: both end in the same band well above -15."""
     mpo_t = ham.heisenberg_xxz_tensors(16, 1.0, 1.0, 0.0)
     mps_t = ct.random_mps_c_compatible_tensors(16, 2, 8, 42)
     p = ct.DMRGParams(num_sweeps=4, chi_max=32, eps_trunc=1e-10, max_krylov_iter=20, krylov_tol=1e-10,
@@ -356,8 +365,23 @@ def test_dmrg_config1_reference_jacobi_cap_stalls(ctx, ora):
                                   return_stats=True)
     ref = ora.dmrg_two_site(mps_t, mpo_t, 4, 32, 1e-10, 20, 1e-10, 0.0)
     assert -15.0 + 1e-3 < E < -14.9 and -15.0 + 1e-3 < ref["E"] < -14.9
-    assert abs(E - ref["E"]) < 2e-2
-    assert abs(st.sweep_energies[0] - ref["sweep_E"][0]) < 1e-6 * 15.0
+    assert abs(E - ref["E"]) < 5e-2
+    for e in (st.sweep_energies[0], ref["sweep_E"][0]):
+        assert -15.0 < e < -14.8
+
+
+def test_dmrg_ising_truncated_trajectory_vs_oracle(ctx, ora):
+    """Tilted-field Ising chain (J=1, h=0.3, g=0.9: no symmetry left, so no
+    degenerate singular values), L=14, chi_max=5 and 3 Krylov vectors so that
+    truncation and unconverged local solves shape every sweep: the whole
+    per-sweep energy trajectory must follow the oracle's to 1e-10 relative."""
+    mpo_t = ham.ising_1d_tensors(14, 1.0, 0.3, 0.9)
+    mps_t = ct.random_mps_c_compatible_tensors(14, 2, 4, 7)
+    E, out, st, ref = _run_both(ctx, ora, mpo_t, mps_t, num_sweeps=4, chi_max=5, eps_trunc=1e-12,
+                                max_krylov_iter=3, krylov_tol=1e-12, energy_tol=0.0)
+    assert np.allclose(st.sweep_energies, ref["sweep_E"], rtol=1e-10, atol=0)
+    assert abs(st.sweep_energies[0] - st.sweep_energies[-1]) > 1e-3
+    assert abs(st.trunc_weight_sum - ref["trunc_weight"]) <= 1e-8 * ref["trunc_weight"] + 1e-14
 
 
 def test_dmrg_heisenberg_afm_vs_oracle_and_ed(ctx, ora):
