@@ -30,15 +30,18 @@ __device__ __forceinline__ double block_sum256(double v) {
 // tiles, then walked in groups of GM tile rows (column-major inside a group),
 // so the A row-panels and B column-panels one XCD touches overlap and stay in
 // its L2.
-__device__ __forceinline__ void gemm_tile_coords(int bid, int nb, int tilesM, int tilesN, int& tm, int& tn) {
+__device__ __forceinline__ void gemm_tile_coords(int bid, int nb, int tilesM, int tilesN, int GM, int& tm, int& tn) {
     if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
-    constexpr int GM = 8;
     const int group = bid / (GM * tilesN);
     const int first = group * GM;
     const int gm = min(tilesM - first, GM);
     const int in = bid - group * GM * tilesN;
     tm = first + in % gm;
     tn = in / gm;
+}
+
+__device__ __forceinline__ void gemm_tile_coords(int bid, int nb, int tilesM, int tilesN, int& tm, int& tn) {
+    gemm_tile_coords(bid, nb, tilesM, tilesN, 8, tm, tn);
 }
 
 }  // namespace ct

@@ -12,8 +12,17 @@
 // HBM.  The kernel therefore has a single "NT" form C = A * B^T; callers that
 // hold m- or n-contiguous operands split them with a transposing split.
 //
-// Block tile 128x128x32, 256 threads = 4 waves (2x2), 64x64 per wave,
-// LDS double-buffered (2 x 40 KiB).
+// Two tilings share the fragment code:
+//  * 256x256x32 blocks, 512 threads = 8 waves (2 x 4), 128x64 per wave, one
+//    80 KiB LDS stage refilled from registers prefetched during the MFMAs;
+//    optional split-K over blockIdx.z with FP32 partial sums reduced by a
+//    second kernel.  Each staged byte feeds twice the MFMA work of the small
+//    tiling, which halves the L2/HBM operand traffic per FLOP.
+//  * 128x128x32 blocks, 256 threads = 4 waves (2x2), 64x64 per wave, LDS
+//    double-buffered (2 x 40 KiB), for problems too small to fill the chip
+//    with 256x256 tiles.
+#include <algorithm>
+
 #include "ct_internal.h"
 #include "ct_device.h"
 
@@ -100,6 +109,10 @@ struct NtArgs {
     int64_t lda, ldb, bsA, bsB, ldc, bsC;
     double alpha, beta;
     const int* active;
+    int GM;           // tile rows per L2 group (see gemm_tile_coords)
+    int splits;       // split-K factor (256x256 tiling)
+    int kchunk;       // K range per split, multiple of BK
+    float* part;      // [batch*splits][M][N] FP32 partial sums when splits > 1
 };
 
 // Each thread copies one 32-byte half row (16 bf16) of each of the four
@@ -232,6 +245,171 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
         }
 }
 
+// ------------------------------------------------------- 256x256 tiling
+constexpr int BB = 256;  // block rows of A and of B
+
+// Thread t copies 32 bytes (16 bf16) of row t % 256, k-half t / 256, of one
+// plane: two 16-byte loads.
+__device__ __forceinline__ void nt_load256(const uint16_t* P, int64_t ld, int R0, int RN, int k0, int kend, bool vec,
+                                           v4u* r) {
+    const int row = R0 + (threadIdx.x & 255), kh = threadIdx.x >> 8;
+    const int kb = k0 + 16 * kh;
+    if (vec && row < RN && kb + 15 < kend) {
+        const v4u* q = (const v4u*)(P + (int64_t)row * ld + kb);
+        r[0] = q[0];
+        r[1] = q[1];
+    } else {
+        uint16_t t[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) t[e] = (row < RN && kb + e < kend) ? P[(int64_t)row * ld + kb + e] : (uint16_t)0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            r[q] = v4u{(uint32_t)t[8 * q + 0] | ((uint32_t)t[8 * q + 1] << 16),
+                       (uint32_t)t[8 * q + 2] | ((uint32_t)t[8 * q + 3] << 16),
+                       (uint32_t)t[8 * q + 4] | ((uint32_t)t[8 * q + 5] << 16),
+                       (uint32_t)t[8 * q + 6] | ((uint32_t)t[8 * q + 7] << 16)};
+    }
+}
+
+__device__ __forceinline__ void nt_store256(uint16_t (*S)[LDK], const v4u* r) {
+    const int row = threadIdx.x & 255, kh = threadIdx.x >> 8;
+    v4u* q = (v4u*)&S[row][16 * kh];
+    q[0] = r[0];
+    q[1] = r[1];
+}
+
+// SPLIT: write FP32 partial sums of this block's K range to p.part instead
+// of alpha*acc + beta*C.
+template <typename TO, bool SPLIT>
+__global__ __launch_bounds__(512) void gemm_bf16x3_nt256_kernel(NtArgs p) {
+    if (p.active && *p.active == 0) return;
+    __shared__ __attribute__((aligned(16))) uint16_t sAh[BB][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sAl[BB][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBh[BB][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBl[BB][LDK];
+
+    int tm, tn;
+    gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, p.GM, tm, tn);
+    const int m0 = tm * BB, n0 = tn * BB;
+    const int z = blockIdx.z;
+    const int b = z / p.splits, sk = z - b * p.splits;
+    const int kbeg = sk * p.kchunk;
+    const int kend = min(p.K, kbeg + p.kchunk);
+    const uint16_t* Ah = p.Ah + (int64_t)b * p.bsA;
+    const uint16_t* Al = p.Al + (int64_t)b * p.bsA;
+    const uint16_t* Bh = p.Bh + (int64_t)b * p.bsB;
+    const uint16_t* Bl = p.Bl + (int64_t)b * p.bsB;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves, 128 x 64 each
+    const int li = lane & 15, lk = lane >> 4;
+
+    v4f acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    v4u rah[2], ral[2], rbh[2], rbl[2];
+    const bool vec = p.vec != 0;
+    if (kbeg < kend) {
+        nt_load256(Ah, p.lda, m0, p.M, kbeg, kend, vec, rah);
+        nt_load256(Al, p.lda, m0, p.M, kbeg, kend, vec, ral);
+        nt_load256(Bh, p.ldb, n0, p.N, kbeg, kend, vec, rbh);
+        nt_load256(Bl, p.ldb, n0, p.N, kbeg, kend, vec, rbl);
+    }
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        nt_store256(sAh, rah);
+        nt_store256(sAl, ral);
+        nt_store256(sBh, rbh);
+        nt_store256(sBl, rbl);
+        __syncthreads();
+        if (k0 + BK < kend) {  // prefetch the next K step into registers
+            nt_load256(Ah, p.lda, m0, p.M, k0 + BK, kend, vec, rah);
+            nt_load256(Al, p.lda, m0, p.M, k0 + BK, kend, vec, ral);
+            nt_load256(Bh, p.ldb, n0, p.N, k0 + BK, kend, vec, rbh);
+            nt_load256(Bl, p.ldb, n0, p.N, k0 + BK, kend, vec, rbl);
+        }
+        v8bf bh[4], bl[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = wn * 64 + j * 16 + li;
+            bh[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBh[col][8 * lk]);
+            bl[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBl[col][8 * lk]);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = wm * 128 + i * 16 + li;
+            const v8bf ah = __builtin_bit_cast(v8bf, *(const v4u*)&sAh[row][8 * lk]);
+            const v8bf al = __builtin_bit_cast(v8bf, *(const v4u*)&sAl[row][8 * lk]);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+
+    if (SPLIT) {
+        float* P = p.part + (int64_t)z * p.M * p.N;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + wn * 64 + j * 16 + li;
+                if (n >= p.N) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm * 128 + i * 16 + lk * 4 + r;
+                    if (m < p.M) P[(int64_t)m * p.N + n] = acc[i][j][r];
+                }
+            }
+    } else {
+        TO* C = (TO*)p.C + (int64_t)b * p.bsC;
+        const double alpha = p.alpha, beta = p.beta;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + wn * 64 + j * 16 + li;
+                if (n >= p.N) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm * 128 + i * 16 + lk * 4 + r;
+                    if (m < p.M) {
+                        TO* c = C + (int64_t)m * p.ldc + n;
+                        double v = alpha * (double)acc[i][j][r];
+                        if (beta != 0.0) v += beta * (double)(*c);
+                        *c = (TO)v;
+                    }
+                }
+            }
+    }
+}
+
+// C[b][m][n] = alpha * sum_s part[b*splits + s][m][n] + beta * C[b][m][n]
+template <typename TO>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restrict__ part, TO* __restrict__ C, int M,
+                                                            int N, int splits, int batch, int64_t ldc, int64_t bsC,
+                                                            double alpha, double beta, const int* active) {
+    if (active && *active == 0) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t total = MN * batch;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = e / MN, r = e - b * MN;
+        const int64_t m = r / N, n = r - m * N;
+        const float* q = part + b * splits * MN + r;
+        float s = 0.f;
+        for (int k = 0; k < splits; ++k) s += q[k * MN];
+        TO* c = C + b * bsC + m * ldc + n;
+        double v = alpha * (double)s;
+        if (beta != 0.0) v += beta * (double)(*c);
+        *c = (TO)v;
+    }
+}
+
 }  // namespace
 
 int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
@@ -263,6 +441,50 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
     p.vec = (A.ld % 8 == 0) && (B.ld % 8 == 0) && (A.bs % 8 == 0) && (B.bs % 8 == 0) && al16(A.hi) &&
             al16(A.lo) && al16(B.hi) && al16(B.lo);
     p.tilesN = (N + BN - 1) / BN;
+    p.GM = 8;
+    p.splits = 1;
+    p.kchunk = K;
+    p.part = nullptr;
+
+    // 256x256 tiling when it still gives at least one block per CU (after
+    // splitting K where K is long), else the 128x128 tiling.
+    const int64_t t256 = (int64_t)((M + BB - 1) / BB) * ((N + BB - 1) / BB) * batch;
+    int splits = 1;
+    while (t256 * splits < 256 && K / (splits * 2) >= 1024) splits *= 2;
+    if (t256 * splits >= 256) {
+        p.tilesM = (M + BB - 1) / BB;
+        p.tilesN = (N + BB - 1) / BB;
+        // Per-XCD share of one z-plane; group tile rows so that share spans
+        // whole rows of tiles (each A panel then leaves HBM once).
+        const int per_xcd = (p.tilesM * p.tilesN) / 8;
+        p.GM = per_xcd >= p.tilesN ? per_xcd / p.tilesN : 1;
+        if (p.GM < 1) p.GM = 1;
+        p.splits = splits;
+        p.kchunk = (int)(((int64_t)K / splits + BK - 1) / BK * BK);
+        dim3 grid(p.tilesM * p.tilesN, 1, batch * splits), block(512);
+        if (splits > 1) {
+            void* w;
+            CT_CHECK(ws_get(h, WS_SPLITK, (size_t)batch * splits * M * N * 4, &w));
+            p.part = (float*)w;
+            hipLaunchKernelGGL((gemm_bf16x3_nt256_kernel<float, true>), grid, block, 0, h->stream, p);
+            const int64_t total = (int64_t)M * N * batch;
+            const int g = (int)std::min<int64_t>((total + 255) / 256, 8192);
+            if (tC == CT_F64)
+                hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(g), dim3(256), 0, h->stream,
+                                   (const float*)p.part, (double*)C, M, N, splits, batch, ldc, bsC, alpha, beta,
+                                   active);
+            else
+                hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(g), dim3(256), 0, h->stream,
+                                   (const float*)p.part, (float*)C, M, N, splits, batch, ldc, bsC, alpha, beta,
+                                   active);
+        } else if (tC == CT_F64) {
+            hipLaunchKernelGGL((gemm_bf16x3_nt256_kernel<double, false>), grid, block, 0, h->stream, p);
+        } else {
+            hipLaunchKernelGGL((gemm_bf16x3_nt256_kernel<float, false>), grid, block, 0, h->stream, p);
+        }
+        CT_HIP(hipGetLastError());
+        return 0;
+    }
     dim3 grid(p.tilesM * p.tilesN, 1, batch), block(256);
     if (tC == CT_F64)
         hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, grid, block, 0, h->stream, p);

@@ -68,7 +68,7 @@ struct Handle {
     bool own_stream = false;
     rocblas_handle blas = nullptr;
     // named workspaces (H_eff intermediates, reductions, SVD, Lanczos basis ...)
-    Buffer ws[24];
+    Buffer ws[32];
     double* d_scalars = nullptr;  // small FP64 scratch for reductions (pinned-size)
     double* h_pinned = nullptr;   // pinned host mirror for scalar read-back
     int num_cu = 256;
@@ -99,7 +99,8 @@ enum WsSlot {
     WS_ENV_T2 = 21,
     WS_ENV_CONV = 22,
     WS_W2 = 23,      // fused MPO coefficients
-    WS_COUNT = 24
+    WS_SPLITK = 24,  // split-K partial sums of the 3xBF16 GEMM
+    WS_COUNT = 25
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);

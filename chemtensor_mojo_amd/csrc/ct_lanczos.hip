@@ -8,6 +8,12 @@
 // the recurrence into no-ops, so no per-iteration read-back is needed (the
 // reference copies every Krylov vector device->host and back, :640-689).
 //
+// Mixed-precision matvecs (no reference counterpart) carry a relative error
+// far above DBL_EPSILON, so for them the breakdown test also fires once
+// beta_j falls below that error level times max_i |alpha_i| (1e-6 for FP32,
+// 1e-5 for 3xBF16): past that point new Krylov directions are matvec noise.
+// The FP64 path keeps the reference test unchanged.
+//
 // Optional refinement for mixed-precision matvecs (no reference counterpart):
 //   refine >= 1: E_rq = <psi|H psi> with an FP64 matvec of the normalised
 //                Ritz vector psi; r = H psi - E_rq psi.
@@ -61,7 +67,7 @@ __global__ __launch_bounds__(256) void lz_dot_partial(const double* __restrict__
 // what: 0 -> scal[idx] = sum; 1 -> alpha[idx] = sum;
 //       2 -> beta step idx: b = sqrt(sum); breakdown test; beta[idx] = b
 __global__ __launch_bounds__(256) void lz_reduce(const double* __restrict__ part, LzState* s, int what, int idx,
-                                                 double thr, int gated) {
+                                                 double thr, int gated, double floor_rel = 0.0) {
     if (gated && !s->active) return;
     double acc = 0.0;
     for (int i = threadIdx.x; i < NB; i += blockDim.x) acc += part[i];
@@ -73,7 +79,13 @@ __global__ __launch_bounds__(256) void lz_reduce(const double* __restrict__ part
         s->alpha[idx] = acc;
     } else {
         const double b = sqrt(acc);
-        if (b < thr) {
+        double t = thr;
+        if (floor_rel > 0.0) {
+            double scale = 0.0;
+            for (int i = 0; i <= idx; ++i) scale = fmax(scale, fabs(s->alpha[i]));
+            t = fmax(t, floor_rel * scale);
+        }
+        if (b < t) {
             s->numiter = idx + 1;
             s->active = 0;
         } else {
@@ -308,6 +320,7 @@ int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta
     LzState* s = (LzState*)sb;
     const int* act = &s->active;
     const double thr = 100.0 * (double)dim * 2.2204460492503131e-16;
+    const double floor_rel = env.prec == CT_PREC_FP32 ? 1e-6 : (env.prec == CT_PREC_BF16X3 ? 1e-5 : 0.0);
     hipStream_t st = h->stream;
     const int G = grid_n(dim);
 
@@ -324,7 +337,7 @@ int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta
         hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, j, thr, 1);
         hipLaunchKernelGGL(lz_update, dim3(NB), dim3(256), 0, st, w, (const double*)vj,
                            (const double*)(j > 0 ? vj - dim : vj), dim, j, (const LzState*)s, part);
-        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 2, j, thr, 1);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 2, j, thr, 1, floor_rel);
         hipLaunchKernelGGL(lz_next, dim3(G), dim3(256), 0, st, (const double*)w, vj + dim, dim, j, (const LzState*)s);
     }
     {   // final alpha-only step when no breakdown happened (dmrg.mojo:691-703)

@@ -69,6 +69,46 @@ def test_tensordot_mixed_vs_oracle(ctx, ora, prec, tol):
         assert rel(C.to_numpy(), exp) < tol, (sA, sB)
 
 
+@pytest.mark.parametrize("M,N,K,dt", [
+    (2048, 2048, 4096, torch.float64),  # 256x256 tiling, split-K = 4, FP64 output
+    (4000, 4100, 200, torch.float32),   # 256x256 tiling, ragged M/N/K, FP32 output
+    (300, 190, 77, torch.float64),      # 128x128 tiling, ragged
+])
+def test_gemm_bf16x3_nt_tilings(ctx, M, N, K, dt):
+    """ct_split_bf16 + ct_gemm_bf16x3_nt against FP64 products on the device:
+    the 3-product sum of the split planes (FP32 accumulation error only) and
+    the exact product (adds the dropped lo*lo term and the rounding of lo)."""
+    from chemtensor_mojo_amd._lib import call
+    import ctypes
+    g = torch.Generator(device="cuda").manual_seed(M + N + K)
+    A = torch.randn(M, K, dtype=torch.float64, device="cuda", generator=g)
+    B = torch.randn(N, K, dtype=torch.float64, device="cuda", generator=g)
+    ld = (K + 7) // 8 * 8
+    planes = [torch.zeros(r * ld, dtype=torch.int16, device="cuda") for r in (M, M, N, N)]
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    call("ct_split_bf16", ctx.handle, 0, vp(A), M, K, K, 1, vp(planes[0]), vp(planes[1]), ld)
+    call("ct_split_bf16", ctx.handle, 0, vp(B), N, K, K, 1, vp(planes[2]), vp(planes[3]), ld)
+    C = torch.full((M, N), float("nan"), dtype=dt, device="cuda")
+    call("ct_gemm_bf16x3_nt", ctx.handle, vp(planes[0]), vp(planes[1]), ld, vp(planes[2]), vp(planes[3]), ld,
+         0 if dt == torch.float64 else 1, vp(C), M, N, K, N)
+    torch.cuda.synchronize()
+
+    def unsplit(t, r):  # bf16 bit patterns -> float64 [r][K]
+        u = t.view(r, ld)[:, :K].to(torch.int32) & 0xFFFF
+        return (u << 16).view(torch.float32).to(torch.float64)
+
+    ah, al, bh, bl = unsplit(planes[0], M), unsplit(planes[1], M), unsplit(planes[2], N), unsplit(planes[3], N)
+    assert float((A - ah - al).abs().max()) < 1e-4
+    three = ah @ bh.T.contiguous() + ah @ bl.T.contiguous() + al @ bh.T.contiguous()
+    exact = A @ B.T.contiguous()
+    Cd = C.to(torch.float64)
+    assert not torch.isnan(Cd).any()
+    r3 = float(torch.linalg.norm(Cd - three) / torch.linalg.norm(three))
+    re = float(torch.linalg.norm(Cd - exact) / torch.linalg.norm(exact))
+    assert r3 < (2e-6 if dt == torch.float64 else 4e-6), r3
+    assert re < 2e-5, re
+
+
 def test_tensordot_integer_kats(ctx):
     """Known answers of test_c_replica_dot.mojo:34-163."""
     def run(a, sa, b, sb, sc):
