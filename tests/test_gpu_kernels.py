@@ -358,6 +358,7 @@ def test_theta_split_vs_oracle(ctx, ora, mode, l2r):
 # ------------------------------------------------------------ full DMRG
 def _run_both(ctx, ora, mpo_t, mps_t, **kw):
     p = ct.DMRGParams(**kw)
+    kw.pop("svd_mode", None)
     mpo = ct.mpo_from_tensors(ctx, mpo_t)
     mps = ct.mps_from_tensors(ctx, mps_t)
     E, out, st = ct.dmrg_two_site(ctx, mpo, mps, p, return_stats=True)
@@ -410,7 +411,8 @@ def test_dmrg_config1_reference_jacobi_cap_stalls(ctx, ora):
         assert -15.0 < e < -14.8
 
 
-def test_dmrg_ising_truncated_trajectory_vs_oracle(ctx, ora):
+@pytest.mark.parametrize("svd_mode", [ct.SVD_GESVD, ct.SVD_GRAM])
+def test_dmrg_ising_truncated_trajectory_vs_oracle(ctx, ora, svd_mode):
     """Tilted-field Ising chain (J=1, h=0.3, g=0.9: no symmetry left, so no
     degenerate singular values), L=14, chi_max=5 and 3 Krylov vectors so that
     truncation and unconverged local solves shape every sweep: the whole
@@ -418,18 +420,19 @@ def test_dmrg_ising_truncated_trajectory_vs_oracle(ctx, ora):
     mpo_t = ham.ising_1d_tensors(14, 1.0, 0.3, 0.9)
     mps_t = ct.random_mps_c_compatible_tensors(14, 2, 4, 7)
     E, out, st, ref = _run_both(ctx, ora, mpo_t, mps_t, num_sweeps=4, chi_max=5, eps_trunc=1e-12,
-                                max_krylov_iter=3, krylov_tol=1e-12, energy_tol=0.0)
+                                max_krylov_iter=3, krylov_tol=1e-12, energy_tol=0.0, svd_mode=svd_mode)
     assert np.allclose(st.sweep_energies, ref["sweep_E"], rtol=1e-10, atol=0)
     assert abs(st.sweep_energies[0] - st.sweep_energies[-1]) > 1e-3
     assert abs(st.trunc_weight_sum - ref["trunc_weight"]) <= 1e-8 * ref["trunc_weight"] + 1e-14
 
 
-def test_dmrg_heisenberg_afm_vs_oracle_and_ed(ctx, ora):
+@pytest.mark.parametrize("svd_mode", [ct.SVD_GESVD, ct.SVD_GRAM])
+def test_dmrg_heisenberg_afm_vs_oracle_and_ed(ctx, ora, svd_mode):
     L = 10
     mpo_t = ham.heisenberg_tensors(L, 1.0)
     mps_t = ct.random_mps_c_compatible_tensors(L, 2, 8, 43)
     E, out, st, ref = _run_both(ctx, ora, mpo_t, mps_t, num_sweeps=4, chi_max=32, eps_trunc=1e-12,
-                                max_krylov_iter=10, krylov_tol=1e-12, energy_tol=0.0)
+                                max_krylov_iter=10, krylov_tol=1e-12, energy_tol=0.0, svd_mode=svd_mode)
     H = ham.mpo_to_dense(mpo_t)
     e0 = np.linalg.eigvalsh(H)[0]
     assert abs(E - ref["E"]) <= 1e-10 * abs(ref["E"])
