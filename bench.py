@@ -189,6 +189,172 @@ def bench_sweep(ctx, L_sites, D, prec, refine, J2=0.5, seed=42, sweeps=2, max_kr
             "max_bond": st.max_bond, "trunc_weight_sum": st.trunc_weight_sum}
 
 
+# ------------------------------------------------------------------ studies
+# Larger workloads run on request (--study NAME); each prints one JSON line.
+
+def _sweep_deltas(run, nsweeps, tag):
+    keys = ("t_lanczos", "t_svd", "t_env", "t_theta", "matvecs", "trunc_weight_sum")
+    prev = {k: 0.0 for k in keys}
+    out = []
+    for s in range(nsweeps):
+        t0 = time.perf_counter()
+        E, _ = run.run(1)
+        wall = time.perf_counter() - t0
+        st = run.stats()
+        cur = {k: float(getattr(st, k)) for k in keys}
+        row = {"sweep": s, "E": E, "s": wall, "max_bond": st.max_bond}
+        row.update({k: cur[k] - prev[k] for k in keys})
+        prev = cur
+        out.append(row)
+        log(f"{tag} sweep {s}: {wall:.3f} s E={E:.14f} maxD={st.max_bond} lanczos={row['t_lanczos']:.3f} "
+            f"svd={row['t_svd']:.3f} env={row['t_env']:.3f} matvecs={int(row['matvecs'])} "
+            f"tw={row['trunc_weight_sum']:.3e}")
+    return out
+
+
+def study_stages(ctx, args):
+    """Plain FP64 DMRG on the GPU next to the FP64 CPU oracle: Heisenberg XXZ
+    chain L=64 (antiferromagnetic sign J=-1 of H = -J sum(XX+YY+Delta ZZ), so the
+    bond dimension grows), Delta=1, w=5, PCG32 random start D=16, chi_max=256,
+    eps_trunc=1e-10, 6 sweeps, 20 Krylov vectors; stage times per sweep.  The
+    oracle runs the first sweep from the same start on all host cores."""
+    import chemtensor_mojo_amd as ct
+    from chemtensor_mojo_amd import hamiltonians as ham
+    L, chi, sweeps = 64, 256, 6
+    mpo_t = ham.heisenberg_xxz_tensors(L, -1.0, 1.0, 0.0)
+    mps_t = ct.random_mps_c_compatible_tensors(L, 2, 16, 42)
+    res = {}
+    for mode, name in ((ct.SVD_GRAM, "gpu_fp64_gram"), (ct.SVD_GESVD, "gpu_fp64_gesvd")):
+        p = ct.DMRGParams(num_sweeps=1, chi_max=chi, eps_trunc=1e-10, max_krylov_iter=20, krylov_tol=1e-10,
+                          energy_tol=0.0, svd_mode=mode)
+        run = ct.DMRGRun(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p)
+        try:
+            res[name] = _sweep_deltas(run, sweeps, name)
+        finally:
+            run.close()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as ora
+        ora.build()
+        ora.set_num_threads(os.cpu_count() or 1)
+        cur, rows = mps_t, []
+        for sw in range(sweeps):  # one sweep per call, each restarting from the previous result
+            t0 = time.perf_counter()
+            ref = ora.dmrg_two_site(cur, mpo_t, 1, chi, 1e-10, 20, 1e-10, 0.0)
+            tc = time.perf_counter() - t0
+            cur = ref["mps"]
+            rows.append({"sweep": sw, "s": tc, "E": float(ref["sweep_E"][0]),
+                         "max_bond": int(max(a.shape[2] for a in cur))})
+            log(f"cpu oracle sweep {sw}: {tc:.2f} s E={rows[-1]['E']:.12f} maxD={rows[-1]['max_bond']}")
+        cpu = {"value": rows[-1]["s"], "unit": "s (last sweep)", "cores": ora.get_num_threads(), "kind": "port",
+               "sample": f"{sweeps} sweeps of the same run, FP64 oracle (OpenBLAS dgemm + LAPACKE dgesdd), "
+                         "one oracle call per sweep", "sweeps": rows}
+    g = res["gpu_fp64_gram"]
+    line = {"study": "fp64_stages", "workload": study_stages.__doc__.split("\n\n")[0].replace("\n", " "),
+            "sweeps": res, "cpu_oracle": cpu,
+            "speedup_last_sweep_vs_cpu": (cpu["value"] / g[-1]["s"]) if cpu else None}
+    print(json.dumps(line), flush=True)
+
+
+def study_precision(ctx, args):
+    """Precision study: J1-J2 chain (J1=1, J2=0.5, w=8) L=100, PCG32 random start
+    D=32, chi_max=1024, eps_trunc=1e-12, 8 sweeps, 20 Krylov vectors, converged
+    tridiagonal solves; FP64 vs FP32 and 3xBF16 matvecs with FP64 refinement.
+    J2=0.5 is the Majumdar-Ghosh point, E0 = -3L/2 = -150 exactly."""
+    import chemtensor_mojo_amd as ct
+    from chemtensor_mojo_amd import hamiltonians as ham
+    L, chi, sweeps = 100, 1024, 8
+    mpo_t = ham.j1j2_tensors(L, 1.0, 0.5)
+    mps_t = ct.random_mps_c_compatible_tensors(L, 2, 32, 42)
+    res = {}
+    for name, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
+                               ("bf16x3_refined", ct.PREC_BF16X3, 2)):
+        p = ct.DMRGParams(num_sweeps=1, chi_max=chi, eps_trunc=1e-12, max_krylov_iter=20, krylov_tol=1e-12,
+                          energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=20000)
+        run = ct.DMRGRun(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p)
+        try:
+            res[name] = _sweep_deltas(run, sweeps, name)
+        finally:
+            run.close()
+    E64 = res["fp64"][-1]["E"]
+    tw64 = sum(r["trunc_weight_sum"] for r in res["fp64"])
+    summary = {}
+    for k, v in res.items():
+        tw = sum(r["trunc_weight_sum"] for r in v)
+        summary[k] = {"E": v[-1]["E"], "dE_rel_vs_fp64": abs(v[-1]["E"] - E64) / abs(E64),
+                      "dE_rel_vs_exact": abs(v[-1]["E"] + 1.5 * L) / (1.5 * L),
+                      "trunc_weight_sum": tw, "d_trunc_weight_vs_fp64": tw - tw64,
+                      "total_s": sum(r["s"] for r in v)}
+        log(f"{k}: E={v[-1]['E']:.14f} dE/E vs fp64 {summary[k]['dE_rel_vs_fp64']:.2e} "
+            f"vs exact {summary[k]['dE_rel_vs_exact']:.2e} tw={tw:.3e} time={summary[k]['total_s']:.2f} s")
+    print(json.dumps({"study": "precision", "workload": study_precision.__doc__.replace("\n", " "),
+                      "summary": summary, "sweeps": res}), flush=True)
+
+
+def study_adjoint(ctx, args):
+    """dE/dJ_b for a random-bond Heisenberg chain: L=64, J_b ~ U[0.5, 1.5]
+    (numpy default_rng(64)), w=5, chi_max=512, FP32-refined forward (6 sweeps
+    from a PCG32 random D=16 start), gradient of all 63 couplings through the
+    adjoint kernels behind torch.autograd; checked against Hellmann-Feynman
+    <psi|dH/dJ_b|psi> on every bond and against central differences (h=1e-5,
+    FP64 sweeps warm-started from the converged MPS) on three bonds."""
+    import torch
+    import chemtensor_mojo_amd as ct
+    from chemtensor_mojo_amd import adjoint as adj
+    from chemtensor_mojo_amd import hamiltonians as ham
+    L, chi = 64, 512
+    J0 = np.random.default_rng(64).uniform(0.5, 1.5, L - 1)
+    mps0 = ct.random_mps_c_compatible_tensors(L, 2, 16, 42)
+    model = adj.random_bond_heisenberg_model(L)
+    times = {}
+    last = {}
+
+    def solve(J):
+        p = ct.DMRGParams(num_sweeps=6, chi_max=chi, eps_trunc=1e-12, max_krylov_iter=20, krylov_tol=1e-12,
+                          energy_tol=0.0, prec=ct.PREC_FP32, refine=2, jacobi_max_rot=20000)
+        E, mps = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, ham.random_bond_heisenberg_tensors(J)),
+                                  ct.mps_from_tensors(ctx, mps0), p)
+        last["mps"] = mps
+        return E, mps
+
+    J = torch.tensor(J0, dtype=torch.float64, requires_grad=True)
+    t0 = time.perf_counter()
+    E = adj.dmrg_energy(J, solve, model, ctx)
+    times["forward_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    E.backward()
+    times["backward_s"] = time.perf_counter() - t0
+    g = J.grad.numpy().copy()
+    log(f"forward {times['forward_s']:.1f} s E={E.item():.12f}; backward {times['backward_s']:.2f} s")
+    # Hellmann-Feynman on the same state: H is linear in J, so dH/dJ_b = H(e_b)
+    mps = last["mps"]
+    t0 = time.perf_counter()
+    hf = np.array([adj.expectation(ctx, mps, ham.random_bond_heisenberg_tensors(np.eye(L - 1)[b]))
+                   for b in range(L - 1)])
+    times["hellmann_feynman_s"] = time.perf_counter() - t0
+    hf_rel = float(np.max(np.abs(g - hf) / np.maximum(1.0, np.abs(hf))))
+    log(f"max |adjoint - HF| / max(1,|HF|) over 63 bonds: {hf_rel:.2e}")
+    # central differences in FP64, warm-started from the converged state
+    fd = {}
+    h = 1e-5
+    for b in (0, 31, 62):
+        vals = []
+        for sgn in (1.0, -1.0):
+            Jb = J0.copy()
+            Jb[b] += sgn * h
+            p = ct.DMRGParams(num_sweeps=4, chi_max=chi, eps_trunc=1e-12, max_krylov_iter=20, krylov_tol=1e-12,
+                              energy_tol=0.0, jacobi_max_rot=20000)
+            Eb, _ = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, ham.random_bond_heisenberg_tensors(Jb)), mps, p)
+            vals.append(Eb)
+        fd[b] = (vals[0] - vals[1]) / (2 * h)
+        log(f"bond {b}: adjoint {g[b]:.10f} HF {hf[b]:.10f} FD {fd[b]:.10f} rel {abs(fd[b] - g[b]) / abs(g[b]):.2e}")
+    print(json.dumps({"study": "adjoint", "workload": study_adjoint.__doc__.replace("\n", " "), "E": E.item(),
+                      "times": times, "max_rel_adjoint_vs_hf": hf_rel,
+                      "fd": {str(b): {"fd": v, "adjoint": float(g[b]), "rel": abs(v - g[b]) / abs(g[b])}
+                             for b, v in fd.items()},
+                      "dEdJ": g.tolist()}), flush=True)
+
+
 def cpu_baseline(D, w, budget_s):
     from oracle import oracle as ora
     ora.build()
@@ -220,6 +386,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--study", choices=("stages", "precision", "adjoint"), default=None,
+                    help="run one of the larger studies instead of the default benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this path runs on one GPU; --gpus must be 1 (multi-GPU sharding is a design note only)",
@@ -231,6 +399,10 @@ def main():
     ctx = ct.DeviceContext(0)
     D, w = args.D, args.w
     log(f"device: {torch.cuda.get_device_name(0)}")
+    if args.study:
+        {"stages": study_stages, "precision": study_precision, "adjoint": study_adjoint}[args.study](ctx, args)
+        ctx.close()
+        return
     mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)
     gm, Fg = bench_dominant_gemm(ctx, D, w, args.steps, args.warmup)
     sweep = None
