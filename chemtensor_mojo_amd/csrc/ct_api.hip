@@ -109,29 +109,28 @@ int ct_tensordot(ct_handle_t h, int prec, int dtype, const void* A, int rankA, c
 }
 
 int ct_split_bf16(ct_handle_t h, int dtype, const void* X, int64_t rows, int64_t K, int64_t s_row, int64_t s_k,
-                  uint16_t* hi, uint16_t* lo, int64_t ld) {
+                  uint16_t* out, int64_t ld) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
     CT_ARG(dtype == CT_F64 || dtype == CT_F32, 2, "unknown dtype");
     CT_ARG(rows >= 0 && rows < (1ll << 31), 4, "rows out of range");
     CT_ARG(K >= 0 && K < (1ll << 31), 5, "K out of range");
-    CT_ARG(ld >= K, 10, "ld must be >= K");
-    return ct::split_bf16(h, dtype, X, (int)rows, (int)K, s_row, s_k, 1, 0, hi, lo, ld, 0, nullptr);
+    CT_ARG(ld >= 2 * ct::split_k32(K), 9, "ld must be >= 2*roundup(K, 32)");
+    return ct::split_bf16(h, dtype, X, (int)rows, (int)K, s_row, s_k, 1, 0, out, ld, 0, nullptr);
 }
 
-int ct_gemm_bf16x3_nt(ct_handle_t h, const uint16_t* A_hi, const uint16_t* A_lo, int64_t lda, const uint16_t* B_hi,
-                      const uint16_t* B_lo, int64_t ldb, int dtype_c, void* C, int64_t M, int64_t N, int64_t K,
-                      int64_t ldc) {
+int ct_gemm_bf16x3_nt(ct_handle_t h, const uint16_t* A, int64_t lda, const uint16_t* B, int64_t ldb, int dtype_c,
+                      void* C, int64_t M, int64_t N, int64_t K, int64_t ldc) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
-    CT_ARG(lda >= K, 4, "lda must be >= K");
-    CT_ARG(ldb >= K, 7, "ldb must be >= K");
-    CT_ARG(dtype_c == CT_F64 || dtype_c == CT_F32, 8, "unknown dtype");
-    CT_ARG(M >= 0 && M < (1ll << 31), 10, "M out of range");
-    CT_ARG(N >= 0 && N < (1ll << 31), 11, "N out of range");
-    CT_ARG(K >= 0 && K < (1ll << 31), 12, "K out of range");
-    CT_ARG(ldc >= N, 13, "ldc must be >= N");
+    CT_ARG(lda >= 2 * ct::split_k32(K), 3, "lda must be >= 2*roundup(K, 32)");
+    CT_ARG(ldb >= 2 * ct::split_k32(K), 5, "ldb must be >= 2*roundup(K, 32)");
+    CT_ARG(dtype_c == CT_F64 || dtype_c == CT_F32, 6, "unknown dtype");
+    CT_ARG(M >= 0 && M < (1ll << 31), 8, "M out of range");
+    CT_ARG(N >= 0 && N < (1ll << 31), 9, "N out of range");
+    CT_ARG(K >= 0 && K < (1ll << 31), 10, "K out of range");
+    CT_ARG(ldc >= N, 11, "ldc must be >= N");
     ct::SplitOp a, b;
-    a.hi = A_hi; a.lo = A_lo; a.ld = lda;
-    b.hi = B_hi; b.lo = B_lo; b.ld = ldb;
+    a.p = A; a.ld = lda;
+    b.p = B; b.ld = ldb;
     return ct::gemm_bf16x3_nt(h, a, b, dtype_c, C, (int)M, (int)N, (int)K, ldc, 0, 1, 1.0, 0.0, nullptr);
 }
 

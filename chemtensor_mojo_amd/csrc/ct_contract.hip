@@ -58,7 +58,6 @@ __global__ void build_env_coef_kernel(const TW* W, int wL, int wR, int d, int ri
         out[(wr * d + s) * (wL * d) + wl * d + so] = v;
 }
 
-inline int64_t pad8(int64_t x) { return (x + 7) / 8 * 8; }
 
 }  // namespace
 
@@ -94,6 +93,11 @@ int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, co
     env->prec = prec;
     env->W2 = d_W2;
     env->dims = hd;
+    {   // padded mixing coefficients, shared by every matvec of this local problem
+        const int tAcc = prec == CT_PREC_FP64 ? CT_F64 : CT_F32;
+        CT_CHECK(mix_pad_coef(h, d_W2, (int)(hd.wL * hd.d * hd.d), (int)(hd.d * hd.d * hd.wR), tAcc,
+                              tAcc == CT_F64 ? WS_W2PAD64 : WS_W2PAD32, &env->W2pad));
+    }
     if (prec == CT_PREC_FP64) {
         if (tEnv != CT_F64) {
             set_last_error("FP64 contraction requires FP64 environments");
@@ -123,23 +127,21 @@ int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, co
         return 0;
     }
     if (prec == CT_PREC_BF16X3) {
-        // Ls: [wL][Dlb][Dl] from L[wL][Dl][Dlb]: row = Dlb (stride 1), k = Dl (stride Dlb)
-        const int64_t ldL = pad8(hd.Dl), bsL = hd.Dlb * ldL, nL = hd.wL * bsL;
-        // Rs: [Drb][wR*Dr] from R[(wR Dr)][Drb]: row = Drb (stride 1), k = wR*Dr (stride Drb)
-        const int64_t KR = hd.wR * hd.Dr, ldR = pad8(KR), nR = hd.Drb * ldR;
+        // Ls: split rows [wL][Dlb][k = Dl] from L[wL][Dl][Dlb]: row stride 1, k stride Dlb
+        const int64_t ldL = 2 * split_k32(hd.Dl), bsL = hd.Dlb * ldL, nL = hd.wL * bsL;
+        // Rs: split rows [Drb][k = wR*Dr] from R[(wR Dr)][Drb]: row stride 1, k stride Drb
+        const int64_t KR = hd.wR * hd.Dr, ldR = 2 * split_k32(KR), nR = hd.Drb * ldR;
         void *ls, *rs;
-        CT_CHECK(ws_get(h, WS_LSPLIT, (size_t)nL * 4, &ls));
-        CT_CHECK(ws_get(h, WS_RSPLIT, (size_t)nR * 4, &rs));
-        uint16_t* lh = (uint16_t*)ls;
-        uint16_t* rh = (uint16_t*)rs;
-        CT_CHECK(split_bf16(h, tEnv, L, (int)hd.Dlb, (int)hd.Dl, 1, hd.Dlb, (int)hd.wL, hd.Dl * hd.Dlb, lh, lh + nL,
-                            ldL, bsL, nullptr));
-        CT_CHECK(split_bf16(h, tEnv, R, (int)hd.Drb, (int)KR, 1, hd.Drb, 1, 0, rh, rh + nR, ldR, 0, nullptr));
+        CT_CHECK(ws_get(h, WS_LSPLIT, (size_t)nL * 2, &ls));
+        CT_CHECK(ws_get(h, WS_RSPLIT, (size_t)nR * 2, &rs));
+        CT_CHECK(split_bf16(h, tEnv, L, (int)hd.Dlb, (int)hd.Dl, 1, hd.Dlb, (int)hd.wL, hd.Dl * hd.Dlb,
+                            (uint16_t*)ls, ldL, bsL, nullptr));
+        CT_CHECK(split_bf16(h, tEnv, R, (int)hd.Drb, (int)KR, 1, hd.Drb, 1, 0, (uint16_t*)rs, ldR, 0, nullptr));
         env->tEnv = tEnv;
         env->L = L;
         env->R = R;
-        env->Ls.hi = lh; env->Ls.lo = lh + nL; env->Ls.ld = ldL; env->Ls.bs = bsL;
-        env->Rs.hi = rh; env->Rs.lo = rh + nR; env->Rs.ld = ldR; env->Rs.bs = 0;
+        env->Ls.p = (const uint16_t*)ls; env->Ls.ld = ldL; env->Ls.bs = bsL;
+        env->Rs.p = (const uint16_t*)rs; env->Rs.ld = ldR; env->Rs.bs = 0;
         return 0;
     }
     set_last_error("unknown precision");
@@ -160,27 +162,27 @@ int heff_apply(Handle* h, const HeffEnv& env, int tVec, const void* theta, void*
     m.active = active;
 
     if (env.prec == CT_PREC_BF16X3) {
-        // theta [Dl][(d d Dr)] -> split [n = d d Dr][k = Dl]
-        const int64_t Nn = d2 * D.Dr, ldV = pad8(D.Dl), nV = Nn * ldV;
+        // theta [Dl][(d d Dr)] -> split rows [n = d d Dr][k = Dl]
+        const int64_t Nn = d2 * D.Dr, ldV = 2 * split_k32(D.Dl), nV = Nn * ldV;
         void *vs, *t1, *us;
-        CT_CHECK(ws_get(h, WS_VSPLIT, (size_t)nV * 4, &vs));
+        CT_CHECK(ws_get(h, WS_VSPLIT, (size_t)nV * 2, &vs));
         CT_CHECK(ws_get(h, WS_T1, (size_t)nT1 * 4, &t1));
-        const int64_t KU = D.wR * D.Dr, ldU = pad8(KU), nUs = D.Dlb * d2 * ldU;
-        CT_CHECK(ws_get(h, WS_USPLIT, (size_t)nUs * 4, &us));
-        uint16_t* vh = (uint16_t*)vs;
-        CT_CHECK(split_bf16(h, tVec, theta, (int)Nn, (int)D.Dl, 1, Nn, 1, 0, vh, vh + nV, ldV, 0, active));
+        const int64_t KU = D.wR * D.Dr, ldU = 2 * split_k32(KU), nUs = D.Dlb * d2 * ldU;
+        CT_CHECK(ws_get(h, WS_USPLIT, (size_t)nUs * 2, &us));
+        CT_CHECK(split_bf16(h, tVec, theta, (int)Nn, (int)D.Dl, 1, Nn, 1, 0, (uint16_t*)vs, ldV, 0, active));
         SplitOp sv;
-        sv.hi = vh; sv.lo = vh + nV; sv.ld = ldV; sv.bs = 0;
+        sv.p = (const uint16_t*)vs; sv.ld = ldV; sv.bs = 0;
         // T1[b][Dlb][(d d Dr)] = Ls[b] (Dlb x Dl) . Vs^T
         CT_CHECK(gemm_bf16x3_nt(h, env.Ls, sv, CT_F32, t1, (int)D.Dlb, (int)Nn, (int)D.Dl, Nn, D.Dlb * Nn,
                                 (int)D.wL, 1.0, 0.0, active));
-        // U written as split planes [(Dlb d d)][ldU] with k = (wR Dr)
+        // U written as split rows [(Dlb d d)][k = (wR Dr)]
         uint16_t* uh = (uint16_t*)us;
         MixDesc ms = m;
         ms.sYc = D.Dr; ms.sYp = d2 * ldU; ms.sYe = ldU;
-        CT_CHECK(mix_split(h, (const float*)t1, uh, nUs, env.W2, ms));
+        CT_CHECK(split_zero_pad(h, uh, D.Dlb * d2, KU, ldU));
+        CT_CHECK(mix_split(h, (const float*)t1, uh, env.W2, ms, env.W2pad));
         SplitOp su;
-        su.hi = uh; su.lo = uh + nUs; su.ld = ldU; su.bs = 0;
+        su.p = uh; su.ld = ldU; su.bs = 0;
         // out[(Dlb d d)][Drb] = Us . Rs^T
         return gemm_bf16x3_nt(h, su, env.Rs, tVec, out, (int)(D.Dlb * d2), (int)D.Drb, (int)KU, D.Drb, 0, 1, 1.0,
                               0.0, active);
@@ -201,7 +203,7 @@ int heff_apply(Handle* h, const HeffEnv& env, int tVec, const void* theta, void*
         CT_CHECK(gemm(h, env.prec, env.tEnv, env.L, tVec, theta, tMid, t1, g));
     }
     // steps 2+3: fused MPO mixing
-    CT_CHECK(mix(h, tMid, t1, tMid, u, env.W2, m));
+    CT_CHECK(mix(h, tMid, t1, tMid, u, env.W2, m, env.W2pad));
     {   // step 4: out[(Dlb d d)][Drb] = U[(Dlb d d)][(wR Dr)] R[(wR Dr)][Drb]
         GemmDesc g;
         g.M = (int)(D.Dlb * d2); g.N = (int)D.Drb; g.K = (int)(D.wR * D.Dr); g.batch = 1;

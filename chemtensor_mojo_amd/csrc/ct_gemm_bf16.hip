@@ -7,20 +7,26 @@
 // dropped al*bl term and the rounding of lo bound the relative error of one
 // product by about 2^-16.
 //
-// The split planes are stored k-contiguous ([row][k]) so an MFMA fragment
-// (8 consecutive k of one row per lane) is one 16-byte read, in LDS and in
-// HBM.  The kernel therefore has a single "NT" form C = A * B^T; callers that
-// hold m- or n-contiguous operands split them with a transposing split.
+// Operand format ("split rows"): row r of an operand holds, for every 32-wide
+// block kb of the contraction index, 32 hi values followed by the 32 lo values
+// of the same k (128 bytes: one cache line).  K is padded with zeros to a
+// multiple of 32, so the kernels need no K-tail code, and an MFMA fragment
+// (8 consecutive k of one row per lane) is one 16-byte read.  The kernels have
+// a single "NT" form C = A * B^T over rows of A (m) and rows of B (n);
+// callers holding m- or n-contiguous data split them with a transposing split.
 //
-// Two tilings share the fragment code:
-//  * 256x256x32 blocks, 512 threads = 8 waves (2 x 4), 128x64 per wave, one
-//    80 KiB LDS stage refilled from registers prefetched during the MFMAs;
-//    optional split-K over blockIdx.z with FP32 partial sums reduced by a
-//    second kernel.  Each staged byte feeds twice the MFMA work of the small
-//    tiling, which halves the L2/HBM operand traffic per FLOP.
-//  * 128x128x32 blocks, 256 threads = 4 waves (2x2), 64x64 per wave, LDS
-//    double-buffered (2 x 40 KiB), for problems too small to fill the chip
-//    with 256x256 tiles.
+// Two tilings:
+//  * 256x256x32 blocks, 256 threads = 4 waves (2x2) of 128x128, accumulators
+//    in AGPRs (one wave per SIMD).  Operand tiles travel HBM/L2 -> LDS by
+//    direct-to-LDS loads (global_load_lds_dwordx4) into two 64 KiB stages: the
+//    next stage is in flight while the MFMAs read the current one, with no
+//    VGPR staging and no LDS write instructions.  Inside a stage the A
+//    fragments of row block i+1 are read while the MFMAs of block i run.
+//    Optional split-K over blockIdx.z with FP32 partial sums reduced by a
+//    second kernel.
+//  * 128x128x32 blocks, 256 threads = 4 waves (2x2) of 64x64, LDS
+//    double-buffered through registers, for problems too small to give every
+//    CU a 256x256 tile.
 #include <algorithm>
 
 #include "ct_internal.h"
@@ -32,6 +38,7 @@ namespace {
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;
 
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
     uint32_t u = __float_as_uint(x);
@@ -49,244 +56,91 @@ __device__ __forceinline__ void split_one(T v, uint16_t& hi, uint16_t& lo) {
 }
 
 // ---------------------------------------------------------------- split
-// Generic strided split.  One thread per (row, k) output element; blocks are
-// 32 (k) x 8 (rows) and stage through LDS so both the strided read and the
-// k-contiguous write are coalesced whichever of s_row / s_k is 1.
+// Strided split into split rows.  One thread per (row, k) of a 32 (k) x 32
+// (rows) block, 256 threads as 32 x 8; the block stages through LDS so both
+// the strided read and the k-contiguous write are coalesced whichever of
+// s_row / s_k is 1.  k in [K, K32) is written as zero.
 template <typename T>
 __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ X, int rows, int K, int64_t s_row,
-                                                    int64_t s_k, int64_t bsX, uint16_t* __restrict__ hi,
-                                                    uint16_t* __restrict__ lo, int64_t ld, int64_t bs,
-                                                    const int* active) {
+                                                    int64_t s_k, int64_t bsX, uint16_t* __restrict__ out, int64_t ld,
+                                                    int64_t bs, const int* active) {
     if (active && *active == 0) return;
     __shared__ float tile[32][33];
     const int b = blockIdx.z;
     const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     X += (int64_t)b * bsX;
+    out += (int64_t)b * bs;
     if (s_row == 1) {
-        // rows contiguous: read with tx along rows
+        // rows contiguous: read with tx along rows, write with tx along k
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            int k = k0 + ty + 8 * i, r = r0 + tx;
+            const int k = k0 + ty + 8 * i, r = r0 + tx;
             tile[ty + 8 * i][tx] = (k < K && r < rows) ? (float)X[(int64_t)r + (int64_t)k * s_k] : 0.f;
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            int r = r0 + ty + 8 * i, k = k0 + tx;
-            if (r < rows && k < K) {
+            const int r = r0 + ty + 8 * i;
+            if (r < rows) {
                 uint16_t h, l;
                 split_one(tile[tx][ty + 8 * i], h, l);
-                int64_t o = (int64_t)b * bs + (int64_t)r * ld + k;
-                hi[o] = h;
-                lo[o] = l;
+                uint16_t* o = out + (int64_t)r * ld + 2 * k0 + tx;
+                o[0] = h;
+                o[32] = l;
             }
         }
     } else {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            int r = r0 + ty + 8 * i, k = k0 + tx;
-            if (r < rows && k < K) {
-                uint16_t h, l;
-                split_one(X[(int64_t)r * s_row + (int64_t)k * s_k], h, l);
-                int64_t o = (int64_t)b * bs + (int64_t)r * ld + k;
-                hi[o] = h;
-                lo[o] = l;
+            const int r = r0 + ty + 8 * i, k = k0 + tx;
+            if (r < rows) {
+                uint16_t h = 0, l = 0;
+                if (k < K) split_one(X[(int64_t)r * s_row + (int64_t)k * s_k], h, l);
+                uint16_t* o = out + (int64_t)r * ld + 2 * k0 + tx;
+                o[0] = h;
+                o[32] = l;
             }
         }
     }
 }
 
-// ------------------------------------------------------------------ GEMM
-constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;  // LDS row = 40 bf16 = 80 B
-
 struct NtArgs {
-    const uint16_t *Ah, *Al, *Bh, *Bl;
+    const uint16_t *A, *B;  // split rows
     void* C;
-    int M, N, K;
+    int M, N, K32;          // K32: padded contraction length (multiple of 32)
     int tilesM, tilesN;
-    int vec;  // 16-byte loads legal (ld % 8 == 0, 16B aligned, K-tail handled separately)
     int64_t lda, ldb, bsA, bsB, ldc, bsC;
     double alpha, beta;
     const int* active;
-    int GM;           // tile rows per L2 group (see gemm_tile_coords)
-    int splits;       // split-K factor (256x256 tiling)
-    int kchunk;       // K range per split, multiple of BK
-    float* part;      // [batch*splits][M][N] FP32 partial sums when splits > 1
+    int GM;                 // tile rows per L2 group (see gemm_tile_coords)
+    int splits;             // split-K factor
+    int kchunk;             // K range per split, multiple of 32
+    float* part;            // [batch*splits][M][N] FP32 partial sums when splits > 1
 };
 
-// Each thread copies one 32-byte half row (16 bf16) of each of the four
-// planes: row = tid % 128, k-half = tid / 128.
-__device__ __forceinline__ void nt_load(const uint16_t* P, int64_t ld, int R0, int RN, int k0, int K, bool vec,
-                                        v4u* r) {
-    const int row = R0 + (threadIdx.x & 127), kh = threadIdx.x >> 7;
-    const int kb = k0 + 16 * kh;
-    if (vec && row < RN && kb + 15 < K) {
-        const v4u* p = (const v4u*)(P + (int64_t)row * ld + kb);
-        r[0] = p[0];
-        r[1] = p[1];
-    } else {
-        uint16_t t[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) t[e] = (row < RN && kb + e < K) ? P[(int64_t)row * ld + kb + e] : (uint16_t)0;
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-            r[q] = v4u{(uint32_t)t[8 * q + 0] | ((uint32_t)t[8 * q + 1] << 16),
-                       (uint32_t)t[8 * q + 2] | ((uint32_t)t[8 * q + 3] << 16),
-                       (uint32_t)t[8 * q + 4] | ((uint32_t)t[8 * q + 5] << 16),
-                       (uint32_t)t[8 * q + 6] | ((uint32_t)t[8 * q + 7] << 16)};
-    }
-}
-
-__device__ __forceinline__ void nt_store(uint16_t (*S)[LDK], const v4u* r) {
-    const int row = threadIdx.x & 127, kh = threadIdx.x >> 7;
-    v4u* p = (v4u*)&S[row][16 * kh];
-    p[0] = r[0];
-    p[1] = r[1];
-}
-
 template <typename TO>
-__global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
-    if (p.active && *p.active == 0) return;
-    __shared__ __attribute__((aligned(16))) uint16_t sAh[2][BM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sAl[2][BM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sBh[2][BN][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sBl[2][BN][LDK];
-
-    int tm, tn;
-    gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
-    const int z = blockIdx.z;
-    const uint16_t* Ah = p.Ah + (int64_t)z * p.bsA;
-    const uint16_t* Al = p.Al + (int64_t)z * p.bsA;
-    const uint16_t* Bh = p.Bh + (int64_t)z * p.bsB;
-    const uint16_t* Bl = p.Bl + (int64_t)z * p.bsB;
-
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int li = lane & 15, lk = lane >> 4;
-
-    v4f acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-
-    v4u rah[2], ral[2], rbh[2], rbl[2];
-    const bool vec = p.vec != 0;
-    nt_load(Ah, p.lda, m0, p.M, 0, p.K, vec, rah);
-    nt_load(Al, p.lda, m0, p.M, 0, p.K, vec, ral);
-    nt_load(Bh, p.ldb, n0, p.N, 0, p.K, vec, rbh);
-    nt_load(Bl, p.ldb, n0, p.N, 0, p.K, vec, rbl);
-    nt_store(sAh[0], rah);
-    nt_store(sAl[0], ral);
-    nt_store(sBh[0], rbh);
-    nt_store(sBl[0], rbl);
-    __syncthreads();
-
-    int buf = 0;
-    for (int k0 = 0; k0 < p.K; k0 += BK) {
-        const bool more = k0 + BK < p.K;
-        if (more) {
-            nt_load(Ah, p.lda, m0, p.M, k0 + BK, p.K, vec, rah);
-            nt_load(Al, p.lda, m0, p.M, k0 + BK, p.K, vec, ral);
-            nt_load(Bh, p.ldb, n0, p.N, k0 + BK, p.K, vec, rbh);
-            nt_load(Bl, p.ldb, n0, p.N, k0 + BK, p.K, vec, rbl);
-        }
-        v8bf ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = wm * 64 + i * 16 + li;
-            ah[i] = __builtin_bit_cast(v8bf, *(const v4u*)&sAh[buf][row][8 * lk]);
-            al[i] = __builtin_bit_cast(v8bf, *(const v4u*)&sAl[buf][row][8 * lk]);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int col = wn * 64 + j * 16 + li;
-            bh[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBh[buf][col][8 * lk]);
-            bl[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBl[buf][col][8 * lk]);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-            }
-        if (more) {
-            nt_store(sAh[buf ^ 1], rah);
-            nt_store(sAl[buf ^ 1], ral);
-            nt_store(sBh[buf ^ 1], rbh);
-            nt_store(sBl[buf ^ 1], rbl);
-        }
-        __syncthreads();
-        buf ^= 1;
+__device__ __forceinline__ void store_out(const NtArgs& p, TO* C, int m, int n, float a) {
+    if (m < p.M && n < p.N) {
+        TO* c = C + (int64_t)m * p.ldc + n;
+        double v = p.alpha * (double)a;
+        if (p.beta != 0.0) v += p.beta * (double)(*c);
+        *c = (TO)v;
     }
-
-    TO* C = (TO*)p.C + (int64_t)z * p.bsC;
-    const double alpha = p.alpha, beta = p.beta;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = n0 + wn * 64 + j * 16 + li;
-            if (n >= p.N) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm * 64 + i * 16 + lk * 4 + r;
-                if (m < p.M) {
-                    TO* c = C + (int64_t)m * p.ldc + n;
-                    double v = alpha * (double)acc[i][j][r];
-                    if (beta != 0.0) v += beta * (double)(*c);
-                    *c = (TO)v;
-                }
-            }
-        }
 }
 
 // ------------------------------------------------------- 256x256 tiling
-constexpr int BB = 256;  // block rows of A and of B
+constexpr int BB = 256, BK = 32;
+constexpr int SROW = 128;             // bytes per row per stage: 32 hi + 32 lo
+constexpr int SOPND = BB * SROW;      // 32 KiB per operand per stage
+constexpr int STAGE = 2 * SOPND;      // 64 KiB
 
-// Thread t copies 32 bytes (16 bf16) of row t % 256, k-half t / 256, of one
-// plane: two 16-byte loads.
-__device__ __forceinline__ void nt_load256(const uint16_t* P, int64_t ld, int R0, int RN, int k0, int kend, bool vec,
-                                           v4u* r) {
-    const int row = R0 + (threadIdx.x & 255), kh = threadIdx.x >> 8;
-    const int kb = k0 + 16 * kh;
-    if (vec && row < RN && kb + 15 < kend) {
-        const v4u* q = (const v4u*)(P + (int64_t)row * ld + kb);
-        r[0] = q[0];
-        r[1] = q[1];
-    } else {
-        uint16_t t[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) t[e] = (row < RN && kb + e < kend) ? P[(int64_t)row * ld + kb + e] : (uint16_t)0;
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-            r[q] = v4u{(uint32_t)t[8 * q + 0] | ((uint32_t)t[8 * q + 1] << 16),
-                       (uint32_t)t[8 * q + 2] | ((uint32_t)t[8 * q + 3] << 16),
-                       (uint32_t)t[8 * q + 4] | ((uint32_t)t[8 * q + 5] << 16),
-                       (uint32_t)t[8 * q + 6] | ((uint32_t)t[8 * q + 7] << 16)};
-    }
-}
-
-__device__ __forceinline__ void nt_store256(uint16_t (*S)[LDK], const v4u* r) {
-    const int row = threadIdx.x & 255, kh = threadIdx.x >> 8;
-    v4u* q = (v4u*)&S[row][16 * kh];
-    q[0] = r[0];
-    q[1] = r[1];
-}
-
-// SPLIT: write FP32 partial sums of this block's K range to p.part instead
-// of alpha*acc + beta*C.
+// SPLIT: write FP32 partial sums of this block's K range to p.part instead of
+// alpha*acc + beta*C.
 template <typename TO, bool SPLIT>
-__global__ __launch_bounds__(512) void gemm_bf16x3_nt256_kernel(NtArgs p) {
+__global__ __launch_bounds__(256) void gemm_bf16x3_dma_kernel(NtArgs p) {
     if (p.active && *p.active == 0) return;
-    __shared__ __attribute__((aligned(16))) uint16_t sAh[BB][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sAl[BB][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sBh[BB][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t sBl[BB][LDK];
+    __shared__ __attribute__((aligned(1024))) char smem[2 * STAGE];
 
     int tm, tn;
     gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, p.GM, tm, tn);
@@ -294,97 +148,107 @@ __global__ __launch_bounds__(512) void gemm_bf16x3_nt256_kernel(NtArgs p) {
     const int z = blockIdx.z;
     const int b = z / p.splits, sk = z - b * p.splits;
     const int kbeg = sk * p.kchunk;
-    const int kend = min(p.K, kbeg + p.kchunk);
-    const uint16_t* Ah = p.Ah + (int64_t)b * p.bsA;
-    const uint16_t* Al = p.Al + (int64_t)b * p.bsA;
-    const uint16_t* Bh = p.Bh + (int64_t)b * p.bsB;
-    const uint16_t* Bl = p.Bl + (int64_t)b * p.bsB;
+    const int kend = min(p.K32, kbeg + p.kchunk);
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 2, wn = wave & 3;  // 2 x 4 waves, 128 x 64 each
+    const int wm = wave >> 1, wn = wave & 1;
     const int li = lane & 15, lk = lane >> 4;
 
-    v4f acc[8][4];
+    // Direct-to-LDS copy: waves 0/1 copy rows 0-127/128-255 of the A tile,
+    // waves 2/3 those of the B tile; one instruction moves 8 rows x 128 B.
+    // Rows past the matrix edge re-read the last row (their results are
+    // never stored).  Inside a row, 16-byte chunk c sits at slot c ^ (r & 7),
+    // so the fragment reads of 16 consecutive rows hit distinct banks.
+    const bool isA = wave < 2;
+    const uint16_t* P = isA ? p.A + (int64_t)b * p.bsA : p.B + (int64_t)b * p.bsB;
+    const int64_t ldp = isA ? p.lda : p.ldb;
+    const int rbase = (isA ? m0 : n0) + 128 * (wave & 1);
+    const int rlast = (isA ? p.M : p.N) - 1;
+    const int lrow = lane >> 3, lslot = lane & 7;
+    const int64_t soff = (int64_t)(isA ? 0 : SOPND) + (wave & 1) * (128 * SROW);
+    auto issue = [&](int stage, int k0) {
+        char* base = smem + stage * STAGE + soff;
+#pragma unroll
+        for (int blk = 0; blk < 16; ++blk) {
+            const int row = blk * 8 + lrow;  // 0..127 within this wave's half
+            const int rr = (wave & 1) * 128 + row;
+            const int chunk = lslot ^ (rr & 7);
+            const int grow = min(rbase + row, rlast);
+            const uint16_t* src = P + (int64_t)grow * ldp + 2 * k0 + chunk * 8;
+            __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)(base + blk * 1024), 16, 0, 0);
+        }
+    };
+
+    v4f acc[8][8];
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < 8; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-    v4u rah[2], ral[2], rbh[2], rbl[2];
-    const bool vec = p.vec != 0;
-    if (kbeg < kend) {
-        nt_load256(Ah, p.lda, m0, p.M, kbeg, kend, vec, rah);
-        nt_load256(Al, p.lda, m0, p.M, kbeg, kend, vec, ral);
-        nt_load256(Bh, p.ldb, n0, p.N, kbeg, kend, vec, rbh);
-        nt_load256(Bl, p.ldb, n0, p.N, kbeg, kend, vec, rbl);
-    }
+    if (kbeg < kend) issue(0, kbeg);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    int stage = 0;
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        nt_store256(sAh, rah);
-        nt_store256(sAl, ral);
-        nt_store256(sBh, rbh);
-        nt_store256(sBl, rbl);
-        __syncthreads();
-        if (k0 + BK < kend) {  // prefetch the next K step into registers
-            nt_load256(Ah, p.lda, m0, p.M, k0 + BK, kend, vec, rah);
-            nt_load256(Al, p.lda, m0, p.M, k0 + BK, kend, vec, ral);
-            nt_load256(Bh, p.ldb, n0, p.N, k0 + BK, kend, vec, rbh);
-            nt_load256(Bl, p.ldb, n0, p.N, k0 + BK, kend, vec, rbl);
-        }
-        v8bf bh[4], bl[4];
+        if (k0 + BK < kend) issue(stage ^ 1, k0 + BK);
+        const char* sa = smem + stage * STAGE;
+        const char* sb = sa + SOPND;
+        auto rdA = [&](int i, v8bf& h, v8bf& l) {
+            const int r = wm * 128 + i * 16 + li;
+            h = __builtin_bit_cast(v8bf, *(const v4u*)(sa + r * SROW + ((lk ^ (r & 7)) * 16)));
+            l = __builtin_bit_cast(v8bf, *(const v4u*)(sa + r * SROW + (((lk + 4) ^ (r & 7)) * 16)));
+        };
+        v8bf ah[2], al[2];
+        rdA(0, ah[0], al[0]);
+        v8bf bh[8], bl[8];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int col = wn * 64 + j * 16 + li;
-            bh[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBh[col][8 * lk]);
-            bl[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBl[col][8 * lk]);
+        for (int j = 0; j < 8; ++j) {
+            const int r = wn * 128 + j * 16 + li;
+            bh[j] = __builtin_bit_cast(v8bf, *(const v4u*)(sb + r * SROW + ((lk ^ (r & 7)) * 16)));
+            bl[j] = __builtin_bit_cast(v8bf, *(const v4u*)(sb + r * SROW + (((lk + 4) ^ (r & 7)) * 16)));
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int row = wm * 128 + i * 16 + li;
-            const v8bf ah = __builtin_bit_cast(v8bf, *(const v4u*)&sAh[row][8 * lk]);
-            const v8bf al = __builtin_bit_cast(v8bf, *(const v4u*)&sAl[row][8 * lk]);
+            if (i + 1 < 8) rdA(i + 1, ah[(i + 1) & 1], al[(i + 1) & 1]);
+            // keep the reads of block i+1 ahead of the MFMAs of block i
+            __builtin_amdgcn_sched_barrier(0);
+            const v8bf h = ah[i & 1], l = al[i & 1];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
-            }
+            for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l, bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h, bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h, bh[j], acc[i][j], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
         }
+        // the next stage has landed in LDS (this wave's copies), then every wave's
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
+        stage ^= 1;
     }
 
     if (SPLIT) {
-        float* P = p.part + (int64_t)z * p.M * p.N;
+        float* Pp = p.part + (int64_t)z * p.M * p.N;
 #pragma unroll
         for (int i = 0; i < 8; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int n = n0 + wn * 64 + j * 16 + li;
-                if (n >= p.N) continue;
+            for (int j = 0; j < 8; ++j) {
+                const int n = n0 + wn * 128 + j * 16 + li;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm * 128 + i * 16 + lk * 4 + r;
-                    if (m < p.M) P[(int64_t)m * p.N + n] = acc[i][j][r];
+                for (int q = 0; q < 4; ++q) {
+                    const int m = m0 + wm * 128 + i * 16 + lk * 4 + q;
+                    if (m < p.M && n < p.N) Pp[(int64_t)m * p.N + n] = acc[i][j][q];
                 }
             }
     } else {
         TO* C = (TO*)p.C + (int64_t)b * p.bsC;
-        const double alpha = p.alpha, beta = p.beta;
 #pragma unroll
         for (int i = 0; i < 8; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int n = n0 + wn * 64 + j * 16 + li;
-                if (n >= p.N) continue;
+            for (int j = 0; j < 8; ++j) {
+                const int n = n0 + wn * 128 + j * 16 + li;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm * 128 + i * 16 + lk * 4 + r;
-                    if (m < p.M) {
-                        TO* c = C + (int64_t)m * p.ldc + n;
-                        double v = alpha * (double)acc[i][j][r];
-                        if (beta != 0.0) v += beta * (double)(*c);
-                        *c = (TO)v;
-                    }
-                }
+                for (int q = 0; q < 4; ++q) store_out(p, C, m0 + wm * 128 + i * 16 + lk * 4 + q, n, acc[i][j][q]);
             }
     }
 }
@@ -410,18 +274,121 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restr
     }
 }
 
+// ------------------------------------------------------- 128x128 tiling
+constexpr int SM = 128, LDK = BK + 8;  // LDS row = 40 bf16 = 80 B (conflict-free padding)
+
+// Thread t copies the 16 hi and the 16 lo values of k-half t / 128 of row
+// t % 128 (four 16-byte loads); rows past the edge re-read the last row.
+__device__ __forceinline__ void nt_load(const uint16_t* P, int64_t ld, int R0, int RN, int k0, v4u* r) {
+    const int row = min(R0 + (int)(threadIdx.x & 127), RN - 1), kh = threadIdx.x >> 7;
+    const v4u* q = (const v4u*)(P + (int64_t)row * ld + 2 * k0 + 16 * kh);
+    r[0] = q[0];
+    r[1] = q[1];
+    r[2] = q[4];  // lo values sit 32 elements (64 B) after the hi ones
+    r[3] = q[5];
+}
+
+__device__ __forceinline__ void nt_store(uint16_t (*Sh)[LDK], uint16_t (*Sl)[LDK], const v4u* r) {
+    const int row = threadIdx.x & 127, kh = threadIdx.x >> 7;
+    v4u* h = (v4u*)&Sh[row][16 * kh];
+    v4u* l = (v4u*)&Sl[row][16 * kh];
+    h[0] = r[0];
+    h[1] = r[1];
+    l[0] = r[2];
+    l[1] = r[3];
+}
+
+template <typename TO>
+__global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
+    if (p.active && *p.active == 0) return;
+    __shared__ __attribute__((aligned(16))) uint16_t sAh[2][SM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sAl[2][SM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBh[2][SM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBl[2][SM][LDK];
+
+    int tm, tn;
+    gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
+    const int m0 = tm * SM, n0 = tn * SM;
+    const int z = blockIdx.z;
+    const uint16_t* A = p.A + (int64_t)z * p.bsA;
+    const uint16_t* B = p.B + (int64_t)z * p.bsB;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    v4f acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    v4u ra[4], rb[4];
+    nt_load(A, p.lda, m0, p.M, 0, ra);
+    nt_load(B, p.ldb, n0, p.N, 0, rb);
+    nt_store(sAh[0], sAl[0], ra);
+    nt_store(sBh[0], sBl[0], rb);
+    __syncthreads();
+
+    int buf = 0;
+    for (int k0 = 0; k0 < p.K32; k0 += BK) {
+        const bool more = k0 + BK < p.K32;
+        if (more) {
+            nt_load(A, p.lda, m0, p.M, k0 + BK, ra);
+            nt_load(B, p.ldb, n0, p.N, k0 + BK, rb);
+        }
+        v8bf ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = wm * 64 + i * 16 + li;
+            ah[i] = __builtin_bit_cast(v8bf, *(const v4u*)&sAh[buf][row][8 * lk]);
+            al[i] = __builtin_bit_cast(v8bf, *(const v4u*)&sAl[buf][row][8 * lk]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = wn * 64 + j * 16 + li;
+            bh[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBh[buf][col][8 * lk]);
+            bl[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBl[buf][col][8 * lk]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        if (more) {
+            nt_store(sAh[buf ^ 1], sAl[buf ^ 1], ra);
+            nt_store(sBh[buf ^ 1], sBl[buf ^ 1], rb);
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+
+    TO* C = (TO*)p.C + (int64_t)z * p.bsC;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + wn * 64 + j * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) store_out(p, C, m0 + wm * 64 + i * 16 + lk * 4 + r, n, acc[i][j][r]);
+        }
+}
+
 }  // namespace
 
 int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
-               int64_t bsX, uint16_t* hi, uint16_t* lo, int64_t ld_out, int64_t bs_out, const int* active) {
+               int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active) {
     if (rows <= 0 || K <= 0 || batch <= 0) return 0;
-    dim3 grid((K + 31) / 32, (rows + 31) / 32, batch), block(256);
+    dim3 grid((unsigned)(split_k32(K) / 32), (unsigned)((rows + 31) / 32), (unsigned)batch), block(256);
     if (tX == CT_F64)
         hipLaunchKernelGGL(split_kernel<double>, grid, block, 0, h->stream, (const double*)X, rows, K, s_row, s_k,
-                           bsX, hi, lo, ld_out, bs_out, active);
+                           bsX, out, ld_out, bs_out, active);
     else
         hipLaunchKernelGGL(split_kernel<float>, grid, block, 0, h->stream, (const float*)X, rows, K, s_row, s_k,
-                           bsX, hi, lo, ld_out, bs_out, active);
+                           bsX, out, ld_out, bs_out, active);
     CT_HIP(hipGetLastError());
     return 0;
 }
@@ -429,28 +396,34 @@ int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row,
 int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* C, int M, int N, int K,
                    int64_t ldc, int64_t bsC, int batch, double alpha, double beta, const int* active) {
     if (M <= 0 || N <= 0 || batch <= 0) return 0;
+    const int K32 = (int)split_k32(K);
+    auto al16 = [](const void* q) { return ((uintptr_t)q % 16) == 0; };
+    if (A.ld < 2 * K32 || B.ld < 2 * K32 || A.ld % 8 || B.ld % 8 || A.bs % 8 || B.bs % 8 || !al16(A.p) ||
+        !al16(B.p)) {
+        set_last_error("3xBF16 GEMM: split rows need ld >= 2*roundup(K,32), ld and batch stride multiples of 8, "
+                       "16-byte aligned bases");
+        return CT_STATUS_SHAPE;
+    }
     NtArgs p;
-    p.Ah = A.hi; p.Al = A.lo; p.Bh = B.hi; p.Bl = B.lo;
+    p.A = A.p; p.B = B.p;
     p.C = C;
-    p.M = M; p.N = N; p.K = K;
-    p.tilesM = (M + BM - 1) / BM;
+    p.M = M; p.N = N; p.K32 = K32;
     p.lda = A.ld; p.ldb = B.ld; p.bsA = A.bs; p.bsB = B.bs;
     p.ldc = ldc; p.bsC = bsC;
     p.alpha = alpha; p.beta = beta; p.active = active;
-    auto al16 = [](const void* q) { return ((uintptr_t)q % 16) == 0; };
-    p.vec = (A.ld % 8 == 0) && (B.ld % 8 == 0) && (A.bs % 8 == 0) && (B.bs % 8 == 0) && al16(A.hi) &&
-            al16(A.lo) && al16(B.hi) && al16(B.lo);
-    p.tilesN = (N + BN - 1) / BN;
     p.GM = 8;
     p.splits = 1;
-    p.kchunk = K;
+    p.kchunk = K32;
     p.part = nullptr;
+    if (K == 0) {  // empty contraction: C = beta * C
+        p.K32 = 0;
+    }
 
-    // 256x256 tiling when it still gives at least one block per CU (after
-    // splitting K where K is long), else the 128x128 tiling.
+    // 256x256 tiling when it still gives every CU a block (after splitting
+    // long K ranges), else the 128x128 tiling.
     const int64_t t256 = (int64_t)((M + BB - 1) / BB) * ((N + BB - 1) / BB) * batch;
     int splits = 1;
-    while (t256 * splits < 256 && K / (splits * 2) >= 1024) splits *= 2;
+    while (t256 * splits < 256 && K32 / (splits * 2) >= 1024) splits *= 2;
     if (t256 * splits >= 256) {
         p.tilesM = (M + BB - 1) / BB;
         p.tilesN = (N + BB - 1) / BB;
@@ -460,13 +433,13 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
         p.GM = per_xcd >= p.tilesN ? per_xcd / p.tilesN : 1;
         if (p.GM < 1) p.GM = 1;
         p.splits = splits;
-        p.kchunk = (int)(((int64_t)K / splits + BK - 1) / BK * BK);
-        dim3 grid(p.tilesM * p.tilesN, 1, batch * splits), block(512);
+        p.kchunk = (K32 / splits + BK - 1) / BK * BK;
+        dim3 grid(p.tilesM * p.tilesN, 1, batch * splits), block(256);
         if (splits > 1) {
             void* w;
             CT_CHECK(ws_get(h, WS_SPLITK, (size_t)batch * splits * M * N * 4, &w));
             p.part = (float*)w;
-            hipLaunchKernelGGL((gemm_bf16x3_nt256_kernel<float, true>), grid, block, 0, h->stream, p);
+            hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<float, true>), grid, block, 0, h->stream, p);
             const int64_t total = (int64_t)M * N * batch;
             const int g = (int)std::min<int64_t>((total + 255) / 256, 8192);
             if (tC == CT_F64)
@@ -478,13 +451,15 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
                                    (const float*)p.part, (float*)C, M, N, splits, batch, ldc, bsC, alpha, beta,
                                    active);
         } else if (tC == CT_F64) {
-            hipLaunchKernelGGL((gemm_bf16x3_nt256_kernel<double, false>), grid, block, 0, h->stream, p);
+            hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<double, false>), grid, block, 0, h->stream, p);
         } else {
-            hipLaunchKernelGGL((gemm_bf16x3_nt256_kernel<float, false>), grid, block, 0, h->stream, p);
+            hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<float, false>), grid, block, 0, h->stream, p);
         }
         CT_HIP(hipGetLastError());
         return 0;
     }
+    p.tilesM = (M + SM - 1) / SM;
+    p.tilesN = (N + SM - 1) / SM;
     dim3 grid(p.tilesM * p.tilesN, 1, batch), block(256);
     if (tC == CT_F64)
         hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, grid, block, 0, h->stream, p);
@@ -496,26 +471,21 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
 
 // Generic entry used by ct_tensordot / the contractions when no pre-split
 // operand is available: split both operands into workspace, then run the NT
-// kernel.  A(m,k) and B(k,n) follow GemmDesc.
+// kernels.  A(m,k) and B(k,n) follow GemmDesc.
 int gemm_bf16x3(Handle* h, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g) {
-    const int64_t K8 = ((int64_t)g.K + 7) / 8 * 8;  // padded row length keeps 16-byte rows
-    const int64_t bsA = (int64_t)g.M * K8, bsB = (int64_t)g.N * K8;
-    const int64_t nA = bsA * g.batch, nB = bsB * g.batch;
+    const int64_t ld = 2 * split_k32(g.K);
+    const int64_t bsA = (int64_t)g.M * ld, bsB = (int64_t)g.N * ld;
     void* wa;
     void* wb;
-    CT_CHECK(ws_get(h, WS_CONV_A, (size_t)nA * 4, &wa));
-    CT_CHECK(ws_get(h, WS_CONV_B, (size_t)nB * 4, &wb));
-    uint16_t* ah = (uint16_t*)wa;
-    uint16_t* al = ah + nA;
-    uint16_t* bh = (uint16_t*)wb;
-    uint16_t* bl = bh + nB;
+    CT_CHECK(ws_get(h, WS_CONV_A, (size_t)bsA * g.batch * 2, &wa));
+    CT_CHECK(ws_get(h, WS_CONV_B, (size_t)bsB * g.batch * 2, &wb));
     // A(m, k): row = m, stride sAm; k stride sAk
-    CT_CHECK(split_bf16(h, tA, A, g.M, g.K, g.sAm, g.sAk, g.batch, g.bsA, ah, al, K8, bsA, g.active));
+    CT_CHECK(split_bf16(h, tA, A, g.M, g.K, g.sAm, g.sAk, g.batch, g.bsA, (uint16_t*)wa, ld, bsA, g.active));
     // B(k, n) -> row = n: stride sBn; k stride sBk
-    CT_CHECK(split_bf16(h, tB, B, g.N, g.K, g.sBn, g.sBk, g.batch, g.bsB, bh, bl, K8, bsB, g.active));
+    CT_CHECK(split_bf16(h, tB, B, g.N, g.K, g.sBn, g.sBk, g.batch, g.bsB, (uint16_t*)wb, ld, bsB, g.active));
     SplitOp sa, sb;
-    sa.hi = ah; sa.lo = al; sa.ld = K8; sa.bs = bsA;
-    sb.hi = bh; sb.lo = bl; sb.ld = K8; sb.bs = bsB;
+    sa.p = (const uint16_t*)wa; sa.ld = ld; sa.bs = bsA;
+    sb.p = (const uint16_t*)wb; sb.ld = ld; sb.bs = bsB;
     return gemm_bf16x3_nt(h, sa, sb, tC, C, g.M, g.N, g.K, g.ldc, g.bsC, g.batch, g.alpha, g.beta, g.active);
 }
 

@@ -100,7 +100,9 @@ enum WsSlot {
     WS_ENV_CONV = 22,
     WS_W2 = 23,      // fused MPO coefficients
     WS_SPLITK = 24,  // split-K partial sums of the 3xBF16 GEMM
-    WS_COUNT = 25
+    WS_W2PAD64 = 25, // zero-padded fused MPO coefficients, FP64 (FP64 H_eff)
+    WS_W2PAD32 = 26, // zero-padded fused MPO coefficients, FP32 (FP32 / 3xBF16 H_eff)
+    WS_COUNT = 27
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);
@@ -126,15 +128,21 @@ int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int 
 
 // 3xBF16 operands pre-split into hi/lo BF16 planes with k-contiguous rows:
 // element (row, k) of batch b at hi[b*bs + row*ld + k] (same for lo).
+// 3xBF16 operand in "split rows": row r of batch b starts at p + b*bs + r*ld;
+// for each 32-wide block kb of the contraction index it holds 32 hi = bf16(x)
+// values followed by the 32 lo = bf16(x - hi) values (one 128-byte line), with
+// k >= K zero-filled up to split_k32(K).  ld >= 2*split_k32(K).
 struct SplitOp {
-    const uint16_t* hi = nullptr;
-    const uint16_t* lo = nullptr;
+    const uint16_t* p = nullptr;
     int64_t ld = 0, bs = 0;
 };
+inline int64_t split_k32(int64_t K) { return (K + 31) / 32 * 32; }
 // Splits X (element (row, k) of batch b at X + b*bsX + row*s_row + k*s_k) into
-// hi = bf16(x), lo = bf16(x - hi) planes of layout [b][row][k] (ld_out, bs_out).
+// split rows at out (ld_out, bs_out), zero-filling k in [K, split_k32(K)).
 int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
-               int64_t bsX, uint16_t* hi, uint16_t* lo, int64_t ld_out, int64_t bs_out, const int* active);
+               int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active);
+// Zeroes k in [K, split_k32(K)) of every row of a split-rows operand.
+int split_zero_pad(Handle* h, uint16_t* out, int64_t rows, int64_t K, int64_t ld);
 // C[b][m][n] = alpha * sum_k (Ah Bh + Ah Bl + Al Bh)[m,k][n,k] + beta * C
 int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* C, int M, int N, int K,
                    int64_t ldc, int64_t bsC, int batch, double alpha, double beta, const int* active);
@@ -149,9 +157,19 @@ struct MixDesc {
     int64_t sYc = 0, sYp = 0, sYe = 0;
     const int* active = nullptr;
 };
-int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef, const MixDesc& m);
-// Same, FP32 input, output written as BF16 hi/lo planes (lo plane at Y + lo_off).
-int mix_split(Handle* h, const float* X, uint16_t* Y, int64_t lo_off, const double* d_coef, const MixDesc& m);
+int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef, const MixDesc& m,
+        const void* padded = nullptr);
+// Writes the zero-padded KM x KM copy of the [K][NO] coefficients that the
+// mixing kernel reads (accumulation type tAcc: CT_F64 for FP64 input, CT_F32
+// otherwise) into workspace `slot`; pass it to mix / mix_split as `padded` to
+// skip the per-call padding.
+int mix_pad_coef(Handle* h, const double* d_coef, int K, int NO, int tAcc, int slot, const void** out);
+// Same, FP32 input, output written as 3xBF16 split rows.
+// Output in split rows: m.sYp / m.sYe give the row offset (a multiple of the
+// row stride) of pixel p / output channel element eo, m.sYc the k offset of
+// output channel co; q adds to k.
+int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, const MixDesc& m,
+              const void* padded = nullptr);
 
 // ------------------------------------------------------------- misc kernels
 int permute(Handle* h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm, void* out);
@@ -181,6 +199,7 @@ struct HeffEnv {
     const void* R = nullptr;
     SplitOp Ls, Rs;
     const double* W2 = nullptr;  // fused MPO coefficients (device, FP64)
+    const void* W2pad = nullptr; // W2 zero-padded in the mixing accumulation type
     HeffDims dims{};
 };
 

@@ -139,6 +139,26 @@ __global__ __launch_bounds__(256) void lz_next(const double* __restrict__ w, dou
 // off-diagonal element (first in row-major order on ties, like the serial
 // scan), stopping when it is below tol.  Writes E0 = smallest diagonal entry
 // (first on ties) and the normalised eigenvector into coef.
+//
+// Lane k owns row k and keeps the maximum of |A[k][j]|, j > k, with its first
+// column; a rotation in the (p, q) plane only changes rows p, q and columns
+// p, q, so a row is rescanned only when it is p or q or its maximum sat in
+// column p or q, and otherwise compares its two changed entries.  The pivot
+// is then one wavefront argmax - the same pivot sequence as a full scan.
+// The rotation angle phi = atan2(2 apq, aqq - app) / 2 in (-pi/2, pi/2] is
+// taken through the half-angle identities: with r = |(aqq - app, 2 apq)|,
+// cos 2phi = (aqq - app) / r and sin 2phi = 2 apq / r; c = cos phi >= 0 and
+// s = sin phi follow from whichever of (1 + cos 2phi) / 2, (1 - cos 2phi) / 2
+// is not a cancellation, and the other from sin 2phi = 2 s c.
+__device__ __forceinline__ void row_scan(const double* A, int n, int k, double& m, int& arg) {
+    m = -1.0;
+    arg = 0x7fffffff;
+    for (int j = k + 1; j < n; ++j) {
+        const double v = fabs(A[k * n + j]);
+        if (v > m) { m = v; arg = j; }
+    }
+}
+
 __global__ __launch_bounds__(64) void lz_jacobi(LzState* s, double tol, int max_rot) {
     __shared__ double A[64 * 64];
     __shared__ double V[64 * 64];
@@ -163,20 +183,12 @@ __global__ __launch_bounds__(64) void lz_jacobi(LzState* s, double tol, int max_
         if (lane == 0) { s->scal[1] = A[0]; s->coef[0] = 1.0; }
         return;
     }
-    const int npairs = n * (n - 1) / 2;
+    double rmax = -1.0;
+    int rarg = 0x7fffffff;
+    if (lane < n) row_scan(A, n, lane, rmax, rarg);
     for (int it = 0; it < max_rot; ++it) {
-        // argmax |A[i][j]|, i < j, first in row-major order
-        double best = -1.0;
-        int bidx = 0x7fffffff;
-        for (int t = lane; t < npairs; t += 64) {
-            // map t -> (i, j) in row-major order of the strict upper triangle
-            int i = 0, rem = t;
-            while (rem >= n - 1 - i) { rem -= n - 1 - i; ++i; }
-            const int j = i + 1 + rem;
-            const double v = fabs(A[i * n + j]);
-            const int lin = i * n + j;
-            if (v > best || (v == best && lin < bidx)) { best = v; bidx = lin; }
-        }
+        double best = rmax;
+        int bidx = rarg < n ? lane * n + rarg : 0x7fffffff;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             const double ov = __shfl_xor(best, off, 64);
@@ -186,8 +198,19 @@ __global__ __launch_bounds__(64) void lz_jacobi(LzState* s, double tol, int max_
         if (best < tol) break;
         const int p = bidx / n, q = bidx % n;
         const double app = A[p * n + p], aqq = A[q * n + q], apq = A[p * n + q];
-        const double phi = 0.5 * atan2(2.0 * apq, aqq - app);
-        const double c = cos(phi), sn = sin(phi);
+        const double x = aqq - app, y = 2.0 * apq;
+        const double r = sqrt(x * x + y * y);
+        double c = 1.0, sn = 0.0;
+        if (r > 0.0) {
+            const double c2 = x / r, s2 = y / r;
+            if (c2 >= 0.0) {
+                c = sqrt(0.5 * (1.0 + c2));
+                sn = s2 / (2.0 * c);
+            } else {
+                sn = copysign(sqrt(0.5 * (1.0 - c2)), y);
+                c = s2 / (2.0 * sn);
+            }
+        }
         __syncthreads();
         for (int k = lane; k < n; k += 64) {
             if (k != p && k != q) {
@@ -208,6 +231,23 @@ __global__ __launch_bounds__(64) void lz_jacobi(LzState* s, double tol, int max_
             A[q * n + p] = 0.0;
         }
         __syncthreads();
+        if (lane < n) {
+            const int k = lane;
+            if (k == p || k == q || rarg == p || rarg == q) {
+                row_scan(A, n, k, rmax, rarg);
+            } else {
+                // k < p < q: both changed entries lie in row k's upper part;
+                // p < k < q: only column q does (A[k][p] is below the diagonal)
+                if (p > k) {
+                    const double v = fabs(A[k * n + p]);
+                    if (v > rmax || (v == rmax && p < rarg)) { rmax = v; rarg = p; }
+                }
+                if (q > k) {
+                    const double v = fabs(A[k * n + q]);
+                    if (v > rmax || (v == rmax && q < rarg)) { rmax = v; rarg = q; }
+                }
+            }
+        }
     }
     if (lane == 0) {
         int mi = 0;

@@ -23,7 +23,6 @@ struct MixArgs {
     int CIN, EIN, COUT, EOUT;
     int64_t P, Q;
     int64_t sXc, sXp, sXe, sYc, sYp, sYe;
-    int64_t lo_off;  // split output: offset (elements) of the lo plane
     const int* active;
 };
 
@@ -46,7 +45,8 @@ __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void
                                                   const TACC* __restrict__ coef, MixArgs a) {
     if (a.active && *a.active == 0) return;
     __shared__ int64_t in_off[KM];
-    __shared__ int64_t out_off[KM];
+    __shared__ int64_t out_off[KM];  // OUT 2: row part of the output offset
+    __shared__ int64_t out_k[KM];    // OUT 2: k index of output channel co
     const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
     for (int k = threadIdx.x; k < KM; k += blockDim.x) {
         const int kk = k < K ? k : K - 1;
@@ -54,7 +54,12 @@ __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void
         in_off[k] = ci * a.sXc + ei * a.sXe;
         const int nn = k < NO ? k : NO - 1;
         const int co = nn / a.EOUT, eo = nn - co * a.EOUT;
-        out_off[k] = co * a.sYc + eo * a.sYe;
+        if (OUT == 2) {
+            out_off[k] = eo * a.sYe;
+            out_k[k] = co * a.sYc;
+        } else {
+            out_off[k] = co * a.sYc + eo * a.sYe;
+        }
     }
     __syncthreads();
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,17 +82,19 @@ __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void
 #pragma unroll
         for (int n = 0; n < KM; ++n) {
             if (n >= NO) break;
-            const int64_t o = out_off[n] + p * a.sYp + q;
             if (OUT == 0) {
-                ((double*)Yv)[o] = (double)acc[n];
+                ((double*)Yv)[out_off[n] + p * a.sYp + q] = (double)acc[n];
             } else if (OUT == 1) {
-                ((float*)Yv)[o] = (float)acc[n];
+                ((float*)Yv)[out_off[n] + p * a.sYp + q] = (float)acc[n];
             } else {
+                // split rows: hi at 2*(k & ~31) + (k & 31), lo 32 elements later
+                const int64_t k = out_k[n] + q;
+                const int64_t o = out_off[n] + p * a.sYp + 2 * (k & ~(int64_t)31) + (k & 31);
                 const float f = (float)acc[n];
                 const uint16_t hb = f2bf(f);
                 const float hf = __uint_as_float(((uint32_t)hb) << 16);
                 ((uint16_t*)Yv)[o] = hb;
-                ((uint16_t*)Yv)[o + a.lo_off] = f2bf(f - hf);
+                ((uint16_t*)Yv)[o + 32] = f2bf(f - hf);
             }
         }
     }
@@ -194,7 +201,7 @@ static inline int grid_for(int64_t n) {
 
 namespace {
 
-int mix_args(const MixDesc& m, int64_t lo_off, MixArgs* a) {
+int mix_args(const MixDesc& m, MixArgs* a) {
     const int K = m.CIN * m.EIN, NO = m.COUT * m.EOUT;
     if (K > MIX_KMAX || NO > MIX_NMAX) {
         set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
@@ -204,26 +211,26 @@ int mix_args(const MixDesc& m, int64_t lo_off, MixArgs* a) {
     a->P = m.P; a->Q = m.Q;
     a->sXc = m.sXc; a->sXp = m.sXp; a->sXe = m.sXe;
     a->sYc = m.sYc; a->sYp = m.sYp; a->sYe = m.sYe;
-    a->lo_off = lo_off;
     a->active = m.active;
     return 0;
 }
 
-static int mix_km(const MixArgs& a) {
-    const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
-    return (K <= 16 && NO <= 16) ? 16 : (K <= 32 && NO <= 32) ? 32 : 64;
-}
+static int mix_km(int K, int NO) { return (K <= 16 && NO <= 16) ? 16 : (K <= 32 && NO <= 32) ? 32 : 64; }
+static int mix_km(const MixArgs& a) { return mix_km(a.CIN * a.EIN, a.COUT * a.EOUT); }
 
-// Padded coefficient block (workspace) in the accumulation type, then the
-// mixing launch.
+// Padded coefficient block (workspace, unless `padded` already holds it) in
+// the accumulation type, then the mixing launch.
 template <typename TX, typename TACC, int OUT>
-int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixArgs& a) {
+int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixArgs& a, const void* padded) {
     const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT, KM = mix_km(a);
-    void* w;
-    CT_CHECK(ws_get(h, WS_MIX, (size_t)KM * KM * sizeof(TACC), &w));
-    TACC* c = (TACC*)w;
-    hipLaunchKernelGGL(pad_coef_kernel<TACC>, dim3((KM * KM + 255) / 256), dim3(256), 0, h->stream, d_coef, c, K,
-                       NO, KM);
+    const TACC* c = (const TACC*)padded;
+    if (!c) {
+        void* w;
+        CT_CHECK(ws_get(h, WS_MIX, (size_t)KM * KM * sizeof(TACC), &w));
+        hipLaunchKernelGGL(pad_coef_kernel<TACC>, dim3((KM * KM + 255) / 256), dim3(256), 0, h->stream, d_coef,
+                           (TACC*)w, K, NO, KM);
+        c = (const TACC*)w;
+    }
     dim3 grid((unsigned)((a.Q + 255) / 256), (unsigned)(a.P < 65535 ? a.P : 65535)), block(256);
     if (KM == 16)
         hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16>), grid, block, 0, h->stream, X, Y, c, a);
@@ -236,18 +243,38 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
 
 }  // namespace
 
-int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef, const MixDesc& m) {
+int mix_pad_coef(Handle* h, const double* d_coef, int K, int NO, int tAcc, int slot, const void** out) {
+    if (K > MIX_KMAX || NO > MIX_NMAX) {
+        set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
+        return CT_STATUS_NOT_SUPPORTED;
+    }
+    const int KM = mix_km(K, NO);
+    void* w;
+    CT_CHECK(ws_get(h, slot, (size_t)KM * KM * (tAcc == CT_F64 ? 8 : 4), &w));
+    if (tAcc == CT_F64)
+        hipLaunchKernelGGL(pad_coef_kernel<double>, dim3((KM * KM + 255) / 256), dim3(256), 0, h->stream, d_coef,
+                           (double*)w, K, NO, KM);
+    else
+        hipLaunchKernelGGL(pad_coef_kernel<float>, dim3((KM * KM + 255) / 256), dim3(256), 0, h->stream, d_coef,
+                           (float*)w, K, NO, KM);
+    CT_HIP(hipGetLastError());
+    *out = w;
+    return 0;
+}
+
+int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef, const MixDesc& m,
+        const void* padded) {
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
-    CT_CHECK(mix_args(m, 0, &a));
+    CT_CHECK(mix_args(m, &a));
     if (tX == CT_F64 && tY == CT_F64) {
-        CT_CHECK((launch_mix<double, double, 0>(h, (const double*)X, Y, d_coef, a)));
+        CT_CHECK((launch_mix<double, double, 0>(h, (const double*)X, Y, d_coef, a, padded)));
     } else if (tX == CT_F64 && tY == CT_F32) {
-        CT_CHECK((launch_mix<double, double, 1>(h, (const double*)X, Y, d_coef, a)));
+        CT_CHECK((launch_mix<double, double, 1>(h, (const double*)X, Y, d_coef, a, padded)));
     } else if (tX == CT_F32 && tY == CT_F32) {
-        CT_CHECK((launch_mix<float, float, 1>(h, (const float*)X, Y, d_coef, a)));
+        CT_CHECK((launch_mix<float, float, 1>(h, (const float*)X, Y, d_coef, a, padded)));
     } else if (tX == CT_F32 && tY == CT_F64) {
-        CT_CHECK((launch_mix<float, float, 0>(h, (const float*)X, Y, d_coef, a)));
+        CT_CHECK((launch_mix<float, float, 0>(h, (const float*)X, Y, d_coef, a, padded)));
     } else {
         set_last_error("mix: unsupported types");
         return CT_STATUS_NOT_SUPPORTED;
@@ -256,13 +283,33 @@ int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef,
     return 0;
 }
 
-// Mixing whose output goes straight into BF16 hi/lo planes (lo plane at
-// Y + lo_off elements); X is FP32.
-int mix_split(Handle* h, const float* X, uint16_t* Y, int64_t lo_off, const double* d_coef, const MixDesc& m) {
+// Mixing whose output goes straight into 3xBF16 split rows; X is FP32.
+int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, const MixDesc& m,
+              const void* padded) {
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
-    CT_CHECK(mix_args(m, lo_off, &a));
-    CT_CHECK((launch_mix<float, float, 2>(h, X, (void*)Y, d_coef, a)));
+    CT_CHECK(mix_args(m, &a));
+    CT_CHECK((launch_mix<float, float, 2>(h, X, (void*)Y, d_coef, a, padded)));
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+__global__ void split_zero_pad_kernel(uint16_t* __restrict__ out, int64_t rows, int64_t K, int64_t npad,
+                                      int64_t ld) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * npad) return;
+    const int64_t r = e / npad, k = K + (e - r * npad);
+    uint16_t* o = out + r * ld + 2 * (k & ~(int64_t)31) + (k & 31);
+    o[0] = 0;
+    o[32] = 0;
+}
+
+int split_zero_pad(Handle* h, uint16_t* out, int64_t rows, int64_t K, int64_t ld) {
+    const int64_t npad = split_k32(K) - K;
+    if (npad == 0 || rows <= 0) return 0;
+    const int64_t n = rows * npad;
+    hipLaunchKernelGGL(split_zero_pad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, out, rows,
+                       K, npad, ld);
     CT_HIP(hipGetLastError());
     return 0;
 }

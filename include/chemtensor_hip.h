@@ -79,18 +79,20 @@ int ct_tensordot(ct_handle_t h, int prec, int dtype, const void* A, int rankA, c
                  int axrange_B, void* C, int rankC, const int64_t* shapeC);
 
 /* 3xBF16 building blocks (no reference counterpart: the reference GEMM is
- * float32 only, dense_tensor.mojo:1320-1328).  ct_split_bf16 writes
- * hi = bf16(x) and lo = bf16(x - hi) of the matrix X (element (r, k) at
- * X[r*s_row + k*s_k], dtype CT_F64/CT_F32) as k-contiguous rows of length
- * ld >= K (uint16 bit patterns).  ct_gemm_bf16x3_nt computes
- * C[m][n] = sum_k Ah Bh + Ah Bl + Al Bh over A [M][K] and B [N][K] split
- * planes on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; C is CT_F64 or
- * CT_F32 with row stride ldc. */
+ * float32 only, dense_tensor.mojo:1320-1328).  Operands are "split rows"
+ * (uint16 BF16 bit patterns): row r starts at out + r*ld and holds, for each
+ * 32-wide block of k, the 32 values hi = bf16(x) followed by the 32 values
+ * lo = bf16(x - hi); k is zero-padded to a multiple of 32, and
+ * ld >= 2*roundup(K, 32).  ct_split_bf16 writes the split rows of the matrix
+ * X (element (r, k) at X[r*s_row + k*s_k], dtype CT_F64/CT_F32).
+ * ct_gemm_bf16x3_nt computes C[m][n] = sum_k Ah Bh + Ah Bl + Al Bh over split
+ * rows A [M] and B [N] on v_mfma_f32_16x16x32_bf16 with FP32 accumulation; C
+ * is CT_F64 or CT_F32 with row stride ldc.  ld must be a multiple of 8 and
+ * the bases 16-byte aligned. */
 int ct_split_bf16(ct_handle_t h, int dtype, const void* X, int64_t rows, int64_t K, int64_t s_row, int64_t s_k,
-                  uint16_t* hi, uint16_t* lo, int64_t ld);
-int ct_gemm_bf16x3_nt(ct_handle_t h, const uint16_t* A_hi, const uint16_t* A_lo, int64_t lda, const uint16_t* B_hi,
-                      const uint16_t* B_lo, int64_t ldb, int dtype_c, void* C, int64_t M, int64_t N, int64_t K,
-                      int64_t ldc);
+                  uint16_t* out, int64_t ld);
+int ct_gemm_bf16x3_nt(ct_handle_t h, const uint16_t* A, int64_t lda, const uint16_t* B, int64_t ldb, int dtype_c,
+                      void* C, int64_t M, int64_t N, int64_t K, int64_t ldc);
 
 /* DenseTensor.transpose (dense_tensor.mojo:460-545): out axis i = in axis perm[i]; rank <= 8. */
 int ct_permute(ct_handle_t h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm,

@@ -69,35 +69,41 @@ def test_tensordot_mixed_vs_oracle(ctx, ora, prec, tol):
         assert rel(C.to_numpy(), exp) < tol, (sA, sB)
 
 
-@pytest.mark.parametrize("M,N,K,dt", [
-    (2048, 2048, 4096, torch.float64),  # 256x256 tiling, split-K = 4, FP64 output
-    (4000, 4100, 200, torch.float32),   # 256x256 tiling, ragged M/N/K, FP32 output
-    (300, 190, 77, torch.float64),      # 128x128 tiling, ragged
+@pytest.mark.parametrize("M,N,K,dt,extra", [
+    (2048, 2048, 4096, torch.float64, 0),   # 256x256 direct-to-LDS tiling, split-K = 4, FP64 output
+    (4000, 4100, 200, torch.float32, 64),   # 256x256 tiling, ragged M/N, K padded 200 -> 224, ld > 2*K32
+    (300, 190, 77, torch.float64, 0),       # 128x128 tiling, ragged
 ])
-def test_gemm_bf16x3_nt_tilings(ctx, M, N, K, dt):
+def test_gemm_bf16x3_nt_tilings(ctx, M, N, K, dt, extra):
     """ct_split_bf16 + ct_gemm_bf16x3_nt against FP64 products on the device:
-    the 3-product sum of the split planes (FP32 accumulation error only) and
-    the exact product (adds the dropped lo*lo term and the rounding of lo)."""
+    the 3-product sum of the split values (FP32 accumulation error only) and
+    the exact product (adds the dropped lo*lo term and the rounding of lo).
+    The split buffers start as BF16 NaN patterns, so a pad the split fails to
+    zero shows up as NaN in C."""
     from chemtensor_mojo_amd._lib import call
     import ctypes
     g = torch.Generator(device="cuda").manual_seed(M + N + K)
     A = torch.randn(M, K, dtype=torch.float64, device="cuda", generator=g)
     B = torch.randn(N, K, dtype=torch.float64, device="cuda", generator=g)
-    ld = (K + 7) // 8 * 8
-    planes = [torch.zeros(r * ld, dtype=torch.int16, device="cuda") for r in (M, M, N, N)]
+    K32 = (K + 31) // 32 * 32
+    ld = 2 * K32 + extra
+    sa = torch.full((M * ld,), 0x7FC0, dtype=torch.int16, device="cuda")
+    sb = torch.full((N * ld,), 0x7FC0, dtype=torch.int16, device="cuda")
     vp = lambda t: ctypes.c_void_p(t.data_ptr())
-    call("ct_split_bf16", ctx.handle, 0, vp(A), M, K, K, 1, vp(planes[0]), vp(planes[1]), ld)
-    call("ct_split_bf16", ctx.handle, 0, vp(B), N, K, K, 1, vp(planes[2]), vp(planes[3]), ld)
+    call("ct_split_bf16", ctx.handle, 0, vp(A), M, K, K, 1, vp(sa), ld)
+    call("ct_split_bf16", ctx.handle, 0, vp(B), N, K, K, 1, vp(sb), ld)
     C = torch.full((M, N), float("nan"), dtype=dt, device="cuda")
-    call("ct_gemm_bf16x3_nt", ctx.handle, vp(planes[0]), vp(planes[1]), ld, vp(planes[2]), vp(planes[3]), ld,
-         0 if dt == torch.float64 else 1, vp(C), M, N, K, N)
+    call("ct_gemm_bf16x3_nt", ctx.handle, vp(sa), ld, vp(sb), ld, 0 if dt == torch.float64 else 1, vp(C), M, N, K, N)
     torch.cuda.synchronize()
 
-    def unsplit(t, r):  # bf16 bit patterns -> float64 [r][K]
-        u = t.view(r, ld)[:, :K].to(torch.int32) & 0xFFFF
-        return (u << 16).view(torch.float32).to(torch.float64)
+    def unsplit(t, r):  # split rows -> (hi, lo) as float64 [r][K32]
+        u = t.view(r, ld)[:, :2 * K32].reshape(r, K32 // 32, 2, 32).to(torch.int32) & 0xFFFF
+        f = (u << 16).view(torch.float32).to(torch.float64)
+        return f[:, :, 0, :].reshape(r, K32), f[:, :, 1, :].reshape(r, K32)
 
-    ah, al, bh, bl = unsplit(planes[0], M), unsplit(planes[1], M), unsplit(planes[2], N), unsplit(planes[3], N)
+    (ah, al), (bh, bl) = unsplit(sa, M), unsplit(sb, N)
+    assert torch.all(ah[:, K:] == 0) and torch.all(al[:, K:] == 0) and torch.all(bh[:, K:] == 0)
+    ah, al, bh, bl = ah[:, :K], al[:, :K], bh[:, :K], bl[:, :K]
     assert float((A - ah - al).abs().max()) < 1e-4
     three = ah @ bh.T.contiguous() + ah @ bl.T.contiguous() + al @ bh.T.contiguous()
     exact = A @ B.T.contiguous()
@@ -107,6 +113,24 @@ def test_gemm_bf16x3_nt_tilings(ctx, M, N, K, dt):
     re = float(torch.linalg.norm(Cd - exact) / torch.linalg.norm(exact))
     assert r3 < (2e-6 if dt == torch.float64 else 4e-6), r3
     assert re < 2e-5, re
+
+
+def test_heff_bf16x3_large_vs_fp64(ctx):
+    """3xBF16 H_eff at D=1024, w=8 (J1-J2 bulk MPO): both matvec GEMMs take
+    the 256x256 direct-to-LDS tiling (the last one with split-K), against the
+    FP64 path on the same device data."""
+    rng = np.random.default_rng(11)
+    D, w = 1024, 8
+    L = rng.standard_normal((w, D, D)) / np.sqrt(D)
+    R = rng.standard_normal((w, D, D)) / np.sqrt(D)
+    W = ham.j1j2_tensors(6, 1.0, 0.5)[2]
+    th = rng.standard_normal((D, 2, 2, D))
+    tL, tR, tth = dev(ctx, L), dev(ctx, R), dev(ctx, th)
+    tW = ct.MPOSite(dev(ctx, W))
+    o64 = ct.apply_two_site_heff(tth, tL, tR, tW, tW, ctx, prec=ct.PREC_FP64).to_numpy()
+    obf = ct.apply_two_site_heff(tth, tL, tR, tW, tW, ctx, prec=ct.PREC_BF16X3).to_numpy()
+    assert np.isfinite(obf).all()
+    assert rel(obf, o64) < 2e-5
 
 
 def test_tensordot_integer_kats(ctx):
