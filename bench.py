@@ -158,7 +158,7 @@ def bench_sweep(ctx, L_sites, D, prec, refine, J2=0.5, seed=42, sweeps=2, max_kr
     mpo = ct.mpo_from_tensors(ctx, ham.j1j2_tensors(L_sites, 1.0, J2))
     mps = ct.create_random_mps_c_compatible(ctx, L_sites, 2, D, seed)
     p = ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=0.0, max_krylov_iter=max_krylov, krylov_tol=1e-12,
-                      energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=20000, svd_mode=ct.SVD_GRAM)
+                      energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=-1, svd_mode=ct.SVD_GRAM)
     run = ct.DMRGRun(ctx, mpo, mps, p)
     keys = ("t_lanczos", "t_svd", "t_env", "t_theta", "matvecs", "matvec_flops")
     try:
@@ -227,7 +227,7 @@ def study_stages(ctx, args):
     res = {}
     for mode, name in ((ct.SVD_GRAM, "gpu_fp64_gram"), (ct.SVD_GESVD, "gpu_fp64_gesvd")):
         p = ct.DMRGParams(num_sweeps=1, chi_max=chi, eps_trunc=1e-10, max_krylov_iter=20, krylov_tol=1e-10,
-                          energy_tol=0.0, svd_mode=mode, jacobi_max_rot=20000)
+                          energy_tol=0.0, svd_mode=mode, jacobi_max_rot=-1)
         run = ct.DMRGRun(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p)
         try:
             res[name] = _sweep_deltas(run, sweeps, name)
@@ -271,7 +271,7 @@ def study_precision(ctx, args):
     for name, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
                                ("bf16x3_refined", ct.PREC_BF16X3, 2)):
         p = ct.DMRGParams(num_sweeps=1, chi_max=chi, eps_trunc=1e-12, max_krylov_iter=20, krylov_tol=1e-12,
-                          energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=20000)
+                          energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=-1)
         run = ct.DMRGRun(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p)
         try:
             res[name] = _sweep_deltas(run, sweeps, name)
@@ -312,7 +312,7 @@ def study_adjoint(ctx, args):
 
     def solve(J):
         p = ct.DMRGParams(num_sweeps=6, chi_max=chi, eps_trunc=1e-12, max_krylov_iter=20, krylov_tol=1e-12,
-                          energy_tol=0.0, prec=ct.PREC_FP32, refine=2, jacobi_max_rot=20000)
+                          energy_tol=0.0, prec=ct.PREC_FP32, refine=2, jacobi_max_rot=-1)
         E, mps = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, ham.random_bond_heisenberg_tensors(J)),
                                   ct.mps_from_tensors(ctx, mps0), p)
         last["mps"] = mps
@@ -344,7 +344,7 @@ def study_adjoint(ctx, args):
             Jb = J0.copy()
             Jb[b] += sgn * h
             p = ct.DMRGParams(num_sweeps=4, chi_max=chi, eps_trunc=1e-12, max_krylov_iter=20, krylov_tol=1e-12,
-                              energy_tol=0.0, jacobi_max_rot=20000)
+                              energy_tol=0.0, jacobi_max_rot=-1)
             Eb, _ = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, ham.random_bond_heisenberg_tensors(Jb)), mps, p)
             vals.append(Eb)
         fd[b] = (vals[0] - vals[1]) / (2 * h)

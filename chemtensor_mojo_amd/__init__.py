@@ -15,6 +15,8 @@ from .state import (MatrixProductOperator, MatrixProductState, MPOSite, MPSSite,
                     create_product_mps, create_random_mps_c_compatible, expectation_value, mpo_from_tensors,
                     mps_from_tensors, random_mps_c_compatible_tensors, update_left_environment,
                     update_right_environment)
+from .results import (DMRGJsonParams, append_bench_record, bond_entropies, dmrg_results_to_json_string,
+                      load_dmrg_reference, mps_norm, save_dmrg_results_to_json)
 from . import hamiltonians
 
 __version__ = "0.1.0"

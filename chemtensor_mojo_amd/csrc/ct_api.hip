@@ -235,7 +235,7 @@ int ct_lanczos_two_site(ct_handle_t h, int prec, const double* L, const double* 
     CT_CHECK(ct::heff_prepare(h, prec, CT_F64, L, R, (const double*)w2, hd, &env));
     CT_CHECK(ct::heff_prepare(h, CT_PREC_FP64, CT_F64, L, R, (const double*)w2, hd, &env64));
     ct::LanczosResult r;
-    CT_ARG(max_rotations >= 0, 17, "max_rotations must be >= 0");
+    CT_ARG(max_rotations >= -1, 17, "max_rotations must be >= -1");
     CT_CHECK(ct::lanczos(h, env, &env64, theta, max_iter, tol, max_rotations, refine, &r));
     if (E0) *E0 = r.E0;
     if (num_iter) *num_iter = r.num_iter;

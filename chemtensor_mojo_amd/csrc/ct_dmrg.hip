@@ -162,7 +162,7 @@ int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int6
     CT_ARG(params->max_krylov_iter >= 1 && params->max_krylov_iter <= 64, 2, "max_krylov_iter must be in [1, 64]");
     CT_ARG(params->prec >= 0 && params->prec <= 2, 2, "unknown precision");
     CT_ARG(params->env_prec >= 0 && params->env_prec <= 2, 2, "unknown environment precision");
-    CT_ARG(params->jacobi_max_rot >= 0, 2, "jacobi_max_rot must be >= 0");
+    CT_ARG(params->jacobi_max_rot >= -1, 2, "jacobi_max_rot must be >= -1");
     CT_ARG(params->svd_mode == CT_SVD_GESVD || params->svd_mode == CT_SVD_GRAM, 2, "unknown svd_mode");
     ct_dmrg_s* s = new ct_dmrg_s();
     s->h = h;

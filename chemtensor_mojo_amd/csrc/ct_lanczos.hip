@@ -140,64 +140,67 @@ __global__ __launch_bounds__(256) void lz_next(const double* __restrict__ w, dou
 // scan), stopping when it is below tol.  Writes E0 = smallest diagonal entry
 // (first on ties) and the normalised eigenvector into coef.
 //
-// Lane k owns row k and keeps the maximum of |A[k][j]|, j > k, with its first
-// column; a rotation in the (p, q) plane only changes rows p, q and columns
-// p, q, so a row is rescanned only when it is p or q or its maximum sat in
-// column p or q, and otherwise compares its two changed entries.  The pivot
-// is then one wavefront argmax - the same pivot sequence as a full scan.
+// Lane k owns row k.  Before every rotation each lane scans its row's strict
+// upper part with NMAX independent (fully unrolled, predicated) LDS loads and
+// keeps the first maximum.  The pivot is then the largest row maximum, the
+// lowest row on ties (a 64-bit max over the non-negative values' bit patterns
+// followed by a ballot) - the same pivot as a serial row-major scan of the
+// whole triangle.  Rows are stored with an odd stride (ld = n | 1) so that a
+// column access by 32 lanes touches 32 distinct bank pairs.
 // The rotation angle phi = atan2(2 apq, aqq - app) / 2 in (-pi/2, pi/2] is
 // taken through the half-angle identities: with r = |(aqq - app, 2 apq)|,
 // cos 2phi = (aqq - app) / r and sin 2phi = 2 apq / r; c = cos phi >= 0 and
 // s = sin phi follow from whichever of (1 + cos 2phi) / 2, (1 - cos 2phi) / 2
 // is not a cancellation, and the other from sin 2phi = 2 s c.
-__device__ __forceinline__ void row_scan(const double* A, int n, int k, double& m, int& arg) {
+// Lane k reads columns k+1 .. k+NMAX-1 of its row with NMAX-1 unconditional
+// loads (columns past n re-read column n-1; rows past n read unused LDS) and
+// selects branch-free, so the loads issue back to back and are waited for
+// once.  The per-lane bound n - k passes through an empty asm each call: the
+// column masks are then recomputed as vector compares in every rotation
+// instead of being hoisted out of the rotation loop as SGPR masks.
+template <int NMAX>
+__device__ __forceinline__ void row_scan(const double* A, int n, int ld, int k, double& m, int& arg) {
+    int nk = n - k;
+    asm volatile("" : "+v"(nk));
+    double v[NMAX];
+#pragma unroll
+    for (int t = 1; t < NMAX; ++t) v[t] = A[k * ld + min(k + t, n - 1)];
     m = -1.0;
     arg = 0x7fffffff;
-    for (int j = k + 1; j < n; ++j) {
-        const double v = fabs(A[k * n + j]);
-        if (v > m) { m = v; arg = j; }
+#pragma unroll
+    for (int t = 1; t < NMAX; ++t) {
+        const double a = t < nk ? fabs(v[t]) : -1.0;
+        const bool g = a > m;
+        m = g ? a : m;
+        arg = g ? k + t : arg;
     }
 }
 
-__global__ __launch_bounds__(64) void lz_jacobi(LzState* s, double tol, int max_rot) {
-    __shared__ double A[64 * 64];
-    __shared__ double V[64 * 64];
-    if (s->err) return;
-    const int n = s->numiter;
-    const int lane = threadIdx.x;
-    if (n > 64) {  // the host limits max_iter; keep the kernel bounded anyway
-        if (lane == 0) s->err = 2;
-        return;
-    }
-    for (int i = lane; i < n * n; i += 64) {
-        const int r = i / n, c = i % n;
-        double v = 0.0;
-        if (r == c) v = s->alpha[r];
-        else if (c == r + 1) v = s->beta[r];
-        else if (r == c + 1) v = s->beta[c];
-        A[i] = v;
-        V[i] = r == c ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    if (n == 1) {
-        if (lane == 0) { s->scal[1] = A[0]; s->coef[0] = 1.0; }
-        return;
-    }
-    double rmax = -1.0;
-    int rarg = 0x7fffffff;
-    if (lane < n) row_scan(A, n, lane, rmax, rarg);
-    for (int it = 0; it < max_rot; ++it) {
-        double best = rmax;
-        int bidx = rarg < n ? lane * n + rarg : 0x7fffffff;
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(best, off, 64);
-            const int oi = __shfl_xor(bidx, off, 64);
-            if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
-        }
-        if (best < tol) break;
-        const int p = bidx / n, q = bidx % n;
-        const double app = A[p * n + p], aqq = A[q * n + q], apq = A[p * n + q];
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+template <int NMAX>
+__device__ void jacobi_solve(double* A, double* V, int n, int ld, double tol, int max_rot) {
+    const int lane = threadIdx.x;
+    for (int it = 0; it < max_rot; ++it) {
+        double m = -1.0;
+        int rarg = 0x7fffffff;
+        row_scan<NMAX>(A, n, ld, lane, m, rarg);  // lanes >= n: every column masked
+        // m >= 0 on rows with an upper part; -1 (sign bit set) elsewhere -> key 0
+        const unsigned long long key = m >= 0.0 ? (unsigned long long)__double_as_longlong(m) : 0ull;
+        const unsigned long long kmax = wave_max_u64(key);
+        const double best = __longlong_as_double((long long)kmax);
+        if (kmax == 0ull || best < tol) break;
+        const unsigned long long hit = __ballot(key == kmax);
+        const int p = __ffsll((long long)hit) - 1;  // lowest row holding the maximum
+        const int q = __shfl(rarg, p, 64);
+        const double app = A[p * ld + p], aqq = A[q * ld + q], apq = A[p * ld + q];
         const double x = aqq - app, y = 2.0 * apq;
         const double r = sqrt(x * x + y * y);
         double c = 1.0, sn = 0.0;
@@ -214,51 +217,202 @@ __global__ __launch_bounds__(64) void lz_jacobi(LzState* s, double tol, int max_
         __syncthreads();
         for (int k = lane; k < n; k += 64) {
             if (k != p && k != q) {
-                const double akp = A[k * n + p], akq = A[k * n + q];
+                const double akp = A[k * ld + p], akq = A[k * ld + q];
                 const double nkp = c * akp - sn * akq, nkq = sn * akp + c * akq;
-                A[k * n + p] = nkp; A[p * n + k] = nkp;
-                A[k * n + q] = nkq; A[q * n + k] = nkq;
+                A[k * ld + p] = nkp; A[p * ld + k] = nkp;
+                A[k * ld + q] = nkq; A[q * ld + k] = nkq;
             }
-            const double vp = V[k * n + p], vq = V[k * n + q];
-            V[k * n + p] = c * vp - sn * vq;
-            V[k * n + q] = sn * vp + c * vq;
+            const double vp = V[k * ld + p], vq = V[k * ld + q];
+            V[k * ld + p] = c * vp - sn * vq;
+            V[k * ld + q] = sn * vp + c * vq;
         }
         __syncthreads();
         if (lane == 0) {
-            A[p * n + p] = c * c * app - 2.0 * sn * c * apq + sn * sn * aqq;
-            A[q * n + q] = sn * sn * app + 2.0 * sn * c * apq + c * c * aqq;
-            A[p * n + q] = 0.0;
-            A[q * n + p] = 0.0;
+            A[p * ld + p] = c * c * app - 2.0 * sn * c * apq + sn * sn * aqq;
+            A[q * ld + q] = sn * sn * app + 2.0 * sn * c * apq + c * c * aqq;
+            A[p * ld + q] = 0.0;
+            A[q * ld + p] = 0.0;
         }
         __syncthreads();
-        if (lane < n) {
-            const int k = lane;
-            if (k == p || k == q || rarg == p || rarg == q) {
-                row_scan(A, n, k, rmax, rarg);
-            } else {
-                // k < p < q: both changed entries lie in row k's upper part;
-                // p < k < q: only column q does (A[k][p] is below the diagonal)
-                if (p > k) {
-                    const double v = fabs(A[k * n + p]);
-                    if (v > rmax || (v == rmax && p < rarg)) { rmax = v; rarg = p; }
-                }
-                if (q > k) {
-                    const double v = fabs(A[k * n + q]);
-                    if (v > rmax || (v == rmax && q < rarg)) { rmax = v; rarg = q; }
-                }
-            }
-        }
     }
+}
+
+__global__ __launch_bounds__(64) void lz_jacobi(LzState* s, double tol, int max_rot) {
+    __shared__ double A[64 * 65];
+    __shared__ double V[64 * 65];
+    if (s->err) return;
+    const int n = s->numiter;
+    const int lane = threadIdx.x;
+    if (n > 64) {  // the host limits max_iter; keep the kernel bounded anyway
+        if (lane == 0) s->err = 2;
+        return;
+    }
+    const int ld = n | 1;
+    for (int i = lane; i < n * n; i += 64) {
+        const int r = i / n, c = i % n;
+        double v = 0.0;
+        if (r == c) v = s->alpha[r];
+        else if (c == r + 1) v = s->beta[r];
+        else if (r == c + 1) v = s->beta[c];
+        A[r * ld + c] = v;
+        V[r * ld + c] = r == c ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (n == 1) {
+        if (lane == 0) { s->scal[1] = A[0]; s->coef[0] = 1.0; }
+        return;
+    }
+    if (n <= 32)
+        jacobi_solve<32>(A, V, n, ld, tol, max_rot);
+    else
+        jacobi_solve<64>(A, V, n, ld, tol, max_rot);
     if (lane == 0) {
         int mi = 0;
         double mv = A[0];
         for (int i = 1; i < n; ++i)
-            if (A[i * n + i] < mv) { mv = A[i * n + i]; mi = i; }
+            if (A[i * ld + i] < mv) { mv = A[i * ld + i]; mi = i; }
         double nr = 0.0;
-        for (int i = 0; i < n; ++i) nr += V[i * n + mi] * V[i * n + mi];
+        for (int i = 0; i < n; ++i) nr += V[i * ld + mi] * V[i * ld + mi];
         nr = sqrt(nr);
-        for (int i = 0; i < n; ++i) s->coef[i] = nr > 0.0 ? V[i * n + mi] / nr : V[i * n + mi];
+        for (int i = 0; i < n; ++i) s->coef[i] = nr > 0.0 ? V[i * ld + mi] / nr : V[i * ld + mi];
         s->scal[1] = mv;
+    }
+}
+
+// Exact lowest eigenpair of the n x n Lanczos tridiagonal T (no reference
+// counterpart; selected by a negative rotation cap).  It returns what the
+// reference's Jacobi returns once converged, to rounding, in microseconds:
+//  1. Sturm-sequence multisection: each of the 64 lanes counts the
+//     eigenvalues of T below its own point of the current bracket
+//     (LDL^T pivots q_i = (a_i - x) - b_{i-1}^2 / q_{i-1}, negative pivots
+//     counted, tiny pivots replaced by -pivmin); the bracket shrinks 65x per
+//     step around the smallest eigenvalue, starting from the Gershgorin
+//     interval, until it is a few ulps wide.
+//  2. Inverse iteration (one lane): three solves with T - lambda I by
+//     Gaussian elimination with partial pivoting on the tridiagonal band
+//     (U keeps two superdiagonals), normalising between solves; the
+//     eigenvector is returned normalised with a non-negative first entry.
+__device__ int sturm_count(const double* a, const double* b2, int n, double x, double pivmin) {
+    int c = 0;
+    double q = a[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    c += q < 0.0;
+    for (int i = 1; i < n; ++i) {
+        q = (a[i] - x) - b2[i - 1] / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        c += q < 0.0;
+    }
+    return c;
+}
+
+__global__ __launch_bounds__(64) void lz_tridiag_exact(LzState* s) {
+    __shared__ double a[64], b[64], b2[64];
+    __shared__ double u0[64], u1[64], u2[64], ml[64], x[64];
+    __shared__ int sw[64];
+    if (s->err) return;
+    const int n = s->numiter;
+    const int lane = threadIdx.x;
+    if (n > 64) {
+        if (lane == 0) s->err = 2;
+        return;
+    }
+    if (lane < n) {
+        a[lane] = s->alpha[lane];
+        b[lane] = lane + 1 < n ? s->beta[lane] : 0.0;
+        b2[lane] = b[lane] * b[lane];
+    }
+    __syncthreads();
+    if (n == 1) {
+        if (lane == 0) { s->scal[1] = a[0]; s->coef[0] = 1.0; }
+        return;
+    }
+    // Gershgorin interval and the pivot floor (wavefront min / max)
+    double lo = 1e300, hi = -1e300, tn = 0.0;
+    if (lane < n) {
+        const double r = fabs(b[lane]) + (lane > 0 ? fabs(b[lane - 1]) : 0.0);
+        lo = a[lane] - r;
+        hi = a[lane] + r;
+        tn = fabs(a[lane]) + r;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 64));
+        hi = fmax(hi, __shfl_xor(hi, off, 64));
+        tn = fmax(tn, __shfl_xor(tn, off, 64));
+    }
+    const double eps = 2.220446049250313e-16;
+    const double pivmin = fmax(eps * eps * tn * tn, 1e-290);
+    lo -= 2.0 * eps * tn + pivmin;
+    hi += 2.0 * eps * tn + pivmin;
+    // multisection: keep count(lo) == 0 <= count(hi) - 1
+    for (int it = 0; it < 40; ++it) {
+        const double w = hi - lo;
+        if (w <= 4.0 * eps * fmax(fabs(lo), fabs(hi)) + pivmin) break;
+        const double x = lo + w * (double)(lane + 1) / 65.0;
+        const int c = sturm_count(a, b2, n, x, pivmin);
+        const unsigned long long hit = __ballot(c >= 1);
+        if (hit == 0ull) {
+            lo = lo + w * 64.0 / 65.0;  // the smallest eigenvalue is above every point
+        } else {
+            const int f = __ffsll((long long)hit) - 1;
+            const double xf = lo + w * (double)(f + 1) / 65.0;
+            const double xp = f > 0 ? lo + w * (double)f / 65.0 : lo;
+            lo = xp;
+            hi = xf;
+        }
+    }
+    const double lam = 0.5 * (lo + hi);
+    if (lane == 0) {
+        // LU of T - lam I with partial pivoting: row i of U = (u0, u1, u2) at columns i, i+1, i+2
+        // (work arrays in LDS; only this lane touches them)
+        const double tiny = eps * fmax(tn, 1e-300);
+        double c0 = a[0] - lam, c1 = b[0], c2 = 0.0;
+        for (int i = 0; i < n - 1; ++i) {
+            const double n0 = b[i], n1 = a[i + 1] - lam, n2 = (i + 2 < n) ? b[i + 1] : 0.0;
+            double p0, p1, p2, o0, o1, o2;
+            if (fabs(n0) > fabs(c0)) { p0 = n0; p1 = n1; p2 = n2; o0 = c0; o1 = c1; o2 = c2; sw[i] = 1; }
+            else { p0 = c0; p1 = c1; p2 = c2; o0 = n0; o1 = n1; o2 = n2; sw[i] = 0; }
+            if (p0 == 0.0) p0 = tiny;
+            const double m = o0 / p0;
+            u0[i] = p0; u1[i] = p1; u2[i] = p2; ml[i] = m;
+            c0 = o1 - m * p1;
+            c1 = o2 - m * p2;
+            c2 = 0.0;
+        }
+        u0[n - 1] = c0 == 0.0 ? tiny : c0;
+        u1[n - 1] = 0.0;
+        u2[n - 1] = 0.0;
+        if (n >= 2) u2[n - 2] = 0.0;
+        for (int i = 0; i < n; ++i) x[i] = 1.0 + 0.25 * (double)((i * 37) % 11) / 11.0;  // generic start
+        for (int pass = 0; pass < 3; ++pass) {
+            for (int i = 0; i < n - 1; ++i) {
+                if (sw[i]) { const double t = x[i]; x[i] = x[i + 1]; x[i + 1] = t; }
+                x[i + 1] -= ml[i] * x[i];
+            }
+            x[n - 1] /= u0[n - 1];
+            for (int i = n - 2; i >= 0; --i) {
+                double v = x[i] - u1[i] * x[i + 1];
+                if (i + 2 < n) v -= u2[i] * x[i + 2];
+                x[i] = v / u0[i];
+            }
+            double nr = 0.0;
+            for (int i = 0; i < n; ++i) nr = fmax(nr, fabs(x[i]));
+            for (int i = 0; i < n; ++i) x[i] /= nr;
+        }
+        double nr = 0.0;
+        for (int i = 0; i < n; ++i) nr += x[i] * x[i];
+        nr = sqrt(nr);
+        const double sg = x[0] < 0.0 ? -1.0 : 1.0;
+        for (int i = 0; i < n; ++i) s->coef[i] = sg * x[i] / nr;
+        // Rayleigh quotient of the vector: the eigenvalue to rounding
+        double rq = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double t = a[i] * s->coef[i];
+            if (i > 0) t += b[i - 1] * s->coef[i - 1];
+            if (i + 1 < n) t += b[i] * s->coef[i + 1];
+            rq += s->coef[i] * t;
+        }
+        s->scal[1] = rq;
     }
 }
 
@@ -387,7 +541,10 @@ int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta
                            (const LzState*)s, 1);
         hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, max_iter - 1, thr, 1);
     }
-    hipLaunchKernelGGL(lz_jacobi, dim3(1), dim3(64), 0, st, s, tol, max_rot);
+    if (max_rot < 0)
+        hipLaunchKernelGGL(lz_tridiag_exact, dim3(1), dim3(64), 0, st, s);
+    else
+        hipLaunchKernelGGL(lz_jacobi, dim3(1), dim3(64), 0, st, s, tol, max_rot);
     hipLaunchKernelGGL(lz_ritz, dim3(NB), dim3(256), 0, st, (const double*)K, dim, d_theta, (const LzState*)s, part);
     hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 2, thr, 0);
     hipLaunchKernelGGL(lz_normalize, dim3(G), dim3(256), 0, st, d_theta, dim, (const LzState*)s, 2);

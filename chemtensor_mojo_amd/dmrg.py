@@ -30,7 +30,10 @@ class DMRGParams:
     env_prec        precision of the environment updates
     jacobi_max_rot  rotation cap of the Jacobi eigensolve of T; 200 is the
                     reference's value (dmrg.mojo:510), which leaves 20x20 T
-                    matrices unconverged late in a run
+                    matrices unconverged late in a run; -1 solves T exactly
+                    (Sturm multisection + inverse iteration: the converged
+                    Jacobi result to rounding, ~200x faster than converging
+                    the Jacobi)
     svd_mode        SVD_GRAM (default: FP64 eigendecomposition of theta theta^T or
                     theta^T theta, 10-25x faster than a full SVD at D >= 512) or
                     SVD_GESVD (full FP64 SVD, the reference's factorisation).  Both

@@ -135,7 +135,9 @@ int ct_two_site_theta(ct_handle_t h, int prec, int dtype, const void* A_i, const
 /* lanczos_two_site_optimize (src/algorithms/dmrg.mojo:486-756), FP64 Krylov
  * basis on the device, reference variant (no reorthogonalisation, breakdown
  * at beta < 100*dim*DBL_EPSILON, cyclic-max Jacobi on T with tolerance tol and
- * at most max_rotations rotations - the reference's cap is 200, dmrg.mojo:510 -,
+ * at most max_rotations rotations - the reference's cap is 200, dmrg.mojo:510 -;
+ * max_rotations = -1 instead solves T exactly (Sturm multisection + inverse
+ * iteration, the converged Jacobi result to rounding),
  * normalised FP64 Ritz vector).  theta (FP64, [Dl,d,d,Dr]) is the start
  * vector on entry and the Ritz vector on return.
  * L, R, Wi, Wj are FP64.  prec selects the matvec precision; the FP32 and
@@ -181,7 +183,8 @@ typedef struct {
     int max_krylov_iter;   /* DMRGParams.max_krylov_iter */
     double krylov_tol;     /* DMRGParams.krylov_tol (Jacobi tolerance) */
     double energy_tol;     /* DMRGParams.energy_tol */
-    int jacobi_max_rot;    /* Jacobi rotation cap of the local eigensolve (reference: 200, dmrg.mojo:510) */
+    int jacobi_max_rot;    /* Jacobi rotation cap of the local eigensolve (reference: 200, dmrg.mojo:510);
+                              -1: exact tridiagonal eigensolve (converged result) */
     int svd_mode;          /* CT_SVD_GESVD (reference factorisation) or CT_SVD_GRAM */
     int prec;              /* matvec precision CT_PREC_* */
     int refine;            /* 0: reference energies; 1: FP64 Rayleigh quotient; 2: + residual step */

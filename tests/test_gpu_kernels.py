@@ -295,6 +295,26 @@ def test_lanczos_fp64_vs_oracle(ctx, ora, dims):
     assert min(rel(vg, vo), rel(-vg, vo)) < 1e-7
 
 
+@pytest.mark.parametrize("dims", [(4, 4), (12, 20), (40, 33), (64, 64)])
+def test_lanczos_exact_tridiagonal_vs_converged_jacobi(ctx, ora, dims):
+    """max_rotations = -1 (Sturm multisection + inverse iteration on T) against
+    the oracle's Lanczos with a Jacobi run to convergence (20000 rotations):
+    same Krylov dimension, energy to 1e-11 relative, Ritz vector to 1e-7 (the
+    bounds of the reference-cap parity test)."""
+    Dl, Dr = dims
+    rng = np.random.default_rng(Dl * 7 + Dr)
+    L, R, Wi, Wj, th = _sym_problem(rng, Dl, Dr)
+    E_o, v_o, nK_o, _, _ = ora.lanczos_two_site(th, L, R, Wi, Wj, max_iter=20, tol=1e-14, max_rot=20000)
+    E_g, v_g, info = ct.lanczos_two_site_optimize(dev(ctx, th), dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, Wi)),
+                                                  ct.MPOSite(dev(ctx, Wj)), ctx, 20, 1e-14, max_rotations=-1,
+                                                  return_info=True)
+    assert info["num_iter"] == nK_o
+    assert abs(E_g - E_o) <= 1e-11 * max(1.0, abs(E_o))
+    vg = v_g.to_numpy().ravel()
+    vo = v_o.ravel()
+    assert min(rel(vg, vo), rel(-vg, vo)) < 1e-7
+
+
 def test_lanczos_breakdown_matches_oracle(ctx, ora):
     """dim 4 < max_iter: the Krylov space is exhausted and beta falls below 100*dim*eps."""
     rng = np.random.default_rng(9)
