@@ -9,7 +9,7 @@ from ._lib import PREC_BF16X3, PREC_FP32, PREC_FP64, SVD_GESVD, SVD_GRAM
 from .dense_tensor import (DenseTensor, DeviceContext, axpy_in_place, compute_row_major_strides, create_dense_tensor,
                            create_dense_tensor_from_data, create_dense_tensor_uninitialized, dense_tensor_dot,
                            dot_product)
-from .dmrg import (DMRGParams, DMRGRun, DMRGStats, apply_two_site_heff, build_two_site_theta, dense_tensor_svd_trunc,
+from .dmrg import (DMRGParams, DMRGRun, DMRGStats, HeffOperator, apply_two_site_heff, build_two_site_theta, dense_tensor_svd_trunc,
                    dmrg_two_site, lanczos_two_site_optimize, split_two_site_theta_svd)
 from .state import (MatrixProductOperator, MatrixProductState, MPOSite, MPSSite, build_right_environments,
                     create_product_mps, create_random_mps_c_compatible, expectation_value, mpo_from_tensors,

@@ -48,6 +48,9 @@ SIGNATURES = {
     "ct_last_error": [],
     "ct_version": [],
     "ct_tensordot": [_vp, _int, _int, _vp, _int, _i64p, _vp, _int, _i64p, _int, _int, _int, _vp, _int, _i64p],
+    "ct_heff_create": [_vp, _int, _int, _vp, _vp, _vp, _vp] + [_i64] * 8 + [_vp],
+    "ct_heff_apply_prepared": [_vp, _vp, _vp],
+    "ct_heff_destroy": [_vp],
     "ct_split_bf16": [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64],
     "ct_gemm_bf16x3_nt": [_vp, _vp, _i64, _vp, _i64, _int, _vp, _i64, _i64, _i64, _i64],
     "ct_permute": [_vp, _int, _vp, _int, _i64p, _ip, _vp],

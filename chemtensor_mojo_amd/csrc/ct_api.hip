@@ -1,5 +1,7 @@
 // extern "C" entry points of libdmrg_hip.so (declared in include/chemtensor_hip.h)
 // and the handle / workspace management behind them.
+#include <new>
+
 #include "ct_internal.h"
 
 namespace ct {
@@ -8,8 +10,7 @@ static thread_local std::string g_last_error;
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
-int ws_get(Handle* h, int slot, size_t bytes, void** out) {
-    Buffer& b = h->ws[slot];
+int buf_get(Handle* h, Buffer& b, size_t bytes, void** out) {
     if (bytes == 0) bytes = 16;
     if (b.bytes < bytes) {
         // earlier work on the stream may still read the old buffer
@@ -31,6 +32,14 @@ int ws_get(Handle* h, int slot, size_t bytes, void** out) {
     *out = b.ptr;
     return 0;
 }
+
+void buf_free(Buffer& b) {
+    if (b.ptr) (void)hipFree(b.ptr);
+    b.ptr = nullptr;
+    b.bytes = 0;
+}
+
+int ws_get(Handle* h, int slot, size_t bytes, void** out) { return buf_get(h, h->ws[slot], bytes, out); }
 
 }  // namespace ct
 
@@ -187,6 +196,55 @@ int ct_heff_apply(ct_handle_t h, int prec, int dtype, const void* L, const void*
     ct::HeffEnv env;
     CT_CHECK(ct::heff_prepare(h, prec, dtype, L, R, (const double*)w2, hd, &env));
     return ct::heff_apply(h, env, dtype, theta, out, nullptr);
+}
+
+struct ct_heff_s {
+    ct::Handle* h = nullptr;
+    int dtype = CT_F64;
+    ct::HeffEnv env;
+    ct::HeffStore store;
+};
+
+int ct_heff_create(ct_handle_t h, int prec, int dtype, const void* L, const void* Wi, const void* Wj, const void* R,
+                   int64_t Dl, int64_t Dlb, int64_t d, int64_t Dr, int64_t Drb, int64_t wL, int64_t wM, int64_t wR,
+                   ct_heff_t* op) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
+    CT_ARG(Dl > 0 && Dlb > 0 && d > 0 && Dr > 0 && Drb > 0 && wL > 0 && wM > 0 && wR > 0, 8, "dimensions must be > 0");
+    CT_ARG(op != nullptr, 16, "op is NULL");
+    *op = nullptr;
+    ct_heff_s* o = new (std::nothrow) ct_heff_s();
+    if (!o) {
+        ct::set_last_error("ct_heff_create: out of host memory");
+        return CT_STATUS_ALLOC;
+    }
+    o->h = h;
+    o->dtype = dtype;
+    ct::HeffDims hd{Dl, Dlb, d, Dr, Drb, wL, wM, wR};
+    void* w2 = nullptr;
+    int rc = ct::buf_get(h, o->store.w2, (size_t)(wL * d * d * d * d * wR) * 8, &w2);
+    if (rc == 0) rc = ct::heff_build_w2(h, dtype, Wi, Wj, wL, wM, wR, d, (double*)w2);
+    if (rc == 0) rc = ct::heff_prepare(h, prec, dtype, L, R, (const double*)w2, hd, &o->env, &o->store);
+    if (rc != 0) {
+        (void)hipStreamSynchronize(h->stream);
+        delete o;
+        return rc;
+    }
+    *op = o;
+    return 0;
+}
+
+int ct_heff_apply_prepared(ct_heff_t op, const void* theta, void* out) {
+    CT_ARG(op != nullptr, 1, "op is NULL");
+    return ct::heff_apply(op->h, op->env, op->dtype, theta, out, nullptr);
+}
+
+int ct_heff_destroy(ct_heff_t op) {
+    if (!op) return 0;
+    CT_HIP(hipStreamSynchronize(op->h->stream));  // kernels may still read the operator's buffers
+    delete op;
+    return 0;
 }
 
 int ct_env_update_left(ct_handle_t h, int prec, int dtype, const void* L, const void* A, const void* W, void* out,

@@ -89,14 +89,22 @@ static int build_env_coef(Handle* h, int tW, const void* W, int64_t wL, int64_t 
 
 // ------------------------------------------------------------------ H_eff
 int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, const double* d_W2,
-                 const HeffDims& hd, HeffEnv* env) {
+                 const HeffDims& hd, HeffEnv* env, HeffStore* store) {
     env->prec = prec;
     env->W2 = d_W2;
     env->dims = hd;
+    // workspace of the handle, or buffers of the prepared operator
+    auto get = [&](int slot, Buffer* own, size_t bytes, void** out) {
+        return store ? buf_get(h, *own, bytes, out) : ws_get(h, slot, bytes, out);
+    };
     {   // padded mixing coefficients, shared by every matvec of this local problem
         const int tAcc = prec == CT_PREC_FP64 ? CT_F64 : CT_F32;
-        CT_CHECK(mix_pad_coef(h, d_W2, (int)(hd.wL * hd.d * hd.d), (int)(hd.d * hd.d * hd.wR), tAcc,
-                              tAcc == CT_F64 ? WS_W2PAD64 : WS_W2PAD32, &env->W2pad));
+        const int K = (int)(hd.wL * hd.d * hd.d), NO = (int)(hd.d * hd.d * hd.wR);
+        void* pad;
+        CT_CHECK(get(tAcc == CT_F64 ? WS_W2PAD64 : WS_W2PAD32, store ? &store->pad : nullptr,
+                     mix_pad_bytes(K, NO, tAcc), &pad));
+        CT_CHECK(mix_pad_coef(h, d_W2, K, NO, tAcc, pad));
+        env->W2pad = pad;
     }
     if (prec == CT_PREC_FP64) {
         if (tEnv != CT_F64) {
@@ -117,8 +125,8 @@ int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, co
         }
         const int64_t nL = hd.wL * hd.Dl * hd.Dlb, nR = hd.wR * hd.Dr * hd.Drb;
         void *l32, *r32;
-        CT_CHECK(ws_get(h, WS_LSPLIT, (size_t)nL * 4, &l32));
-        CT_CHECK(ws_get(h, WS_RSPLIT, (size_t)nR * 4, &r32));
+        CT_CHECK(get(WS_LSPLIT, store ? &store->l : nullptr, (size_t)nL * 4, &l32));
+        CT_CHECK(get(WS_RSPLIT, store ? &store->r : nullptr, (size_t)nR * 4, &r32));
         CT_CHECK(convert(h, CT_F64, L, CT_F32, l32, nL));
         CT_CHECK(convert(h, CT_F64, R, CT_F32, r32, nR));
         env->tEnv = CT_F32;
@@ -132,8 +140,8 @@ int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, co
         // Rs: split rows [Drb][k = wR*Dr] from R[(wR Dr)][Drb]: row stride 1, k stride Drb
         const int64_t KR = hd.wR * hd.Dr, ldR = 2 * split_k32(KR), nR = hd.Drb * ldR;
         void *ls, *rs;
-        CT_CHECK(ws_get(h, WS_LSPLIT, (size_t)nL * 2, &ls));
-        CT_CHECK(ws_get(h, WS_RSPLIT, (size_t)nR * 2, &rs));
+        CT_CHECK(get(WS_LSPLIT, store ? &store->l : nullptr, (size_t)nL * 2, &ls));
+        CT_CHECK(get(WS_RSPLIT, store ? &store->r : nullptr, (size_t)nR * 2, &rs));
         CT_CHECK(split_bf16(h, tEnv, L, (int)hd.Dlb, (int)hd.Dl, 1, hd.Dlb, (int)hd.wL, hd.Dl * hd.Dlb,
                             (uint16_t*)ls, ldL, bsL, nullptr));
         CT_CHECK(split_bf16(h, tEnv, R, (int)hd.Drb, (int)KR, 1, hd.Drb, 1, 0, (uint16_t*)rs, ldR, 0, nullptr));

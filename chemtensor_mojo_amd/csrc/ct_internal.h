@@ -106,6 +106,10 @@ enum WsSlot {
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);
+// Same growth policy for a buffer owned by another object (synchronises the
+// handle's stream before a reallocation).
+int buf_get(Handle* h, Buffer& b, size_t bytes, void** out);
+void buf_free(Buffer& b);
 
 // ---------------------------------------------------------------- GEMM
 // C[b][m, n] = alpha * sum_k A[b](m, k) * B[b](k, n) + beta * C[b][m, n]
@@ -163,7 +167,8 @@ int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef,
 // mixing kernel reads (accumulation type tAcc: CT_F64 for FP64 input, CT_F32
 // otherwise) into workspace `slot`; pass it to mix / mix_split as `padded` to
 // skip the per-call padding.
-int mix_pad_coef(Handle* h, const double* d_coef, int K, int NO, int tAcc, int slot, const void** out);
+int mix_pad_coef(Handle* h, const double* d_coef, int K, int NO, int tAcc, void* dst);
+size_t mix_pad_bytes(int K, int NO, int tAcc);
 // Same, FP32 input, output written as 3xBF16 split rows.
 // Output in split rows: m.sYp / m.sYe give the row offset (a multiple of the
 // row stride) of pixel p / output channel element eo, m.sYc the k offset of
@@ -211,8 +216,14 @@ int heff_build_w2(Handle* h, int tW, const void* Wi, const void* Wj, int64_t wL,
 // Prepares env for prec.  For 3xBF16 the split planes are written to the
 // WS_LSPLIT / WS_RSPLIT workspaces (valid until the next prepare).  For FP32
 // with FP64 input the converted copies go to the same workspaces.
+// Buffers owned by a prepared H_eff operator (ct_heff_create) instead of the
+// handle's workspaces, so the operator survives other calls on the handle.
+struct HeffStore {
+    Buffer w2, pad, l, r;
+    ~HeffStore() { buf_free(w2); buf_free(pad); buf_free(l); buf_free(r); }
+};
 int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, const double* d_W2,
-                 const HeffDims& hd, HeffEnv* env);
+                 const HeffDims& hd, HeffEnv* env, HeffStore* store = nullptr);
 // out = H_eff theta.  tVec: element type of theta and out.
 int heff_apply(Handle* h, const HeffEnv& env, int tVec, const void* theta, void* out, const int* active);
 

@@ -243,22 +243,24 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
 
 }  // namespace
 
-int mix_pad_coef(Handle* h, const double* d_coef, int K, int NO, int tAcc, int slot, const void** out) {
+size_t mix_pad_bytes(int K, int NO, int tAcc) {
+    const int KM = mix_km(K, NO);
+    return (size_t)KM * KM * (tAcc == CT_F64 ? 8 : 4);
+}
+
+int mix_pad_coef(Handle* h, const double* d_coef, int K, int NO, int tAcc, void* dst) {
     if (K > MIX_KMAX || NO > MIX_NMAX) {
         set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
         return CT_STATUS_NOT_SUPPORTED;
     }
     const int KM = mix_km(K, NO);
-    void* w;
-    CT_CHECK(ws_get(h, slot, (size_t)KM * KM * (tAcc == CT_F64 ? 8 : 4), &w));
     if (tAcc == CT_F64)
         hipLaunchKernelGGL(pad_coef_kernel<double>, dim3((KM * KM + 255) / 256), dim3(256), 0, h->stream, d_coef,
-                           (double*)w, K, NO, KM);
+                           (double*)dst, K, NO, KM);
     else
         hipLaunchKernelGGL(pad_coef_kernel<float>, dim3((KM * KM + 255) / 256), dim3(256), 0, h->stream, d_coef,
-                           (float*)w, K, NO, KM);
+                           (float*)dst, K, NO, KM);
     CT_HIP(hipGetLastError());
-    *out = w;
     return 0;
 }
 

@@ -108,6 +108,57 @@ def apply_two_site_heff(theta: DenseTensor, L_env: DenseTensor, R_env: DenseTens
     return DenseTensor(out, [Dlb, d, d, Drb])
 
 
+class HeffOperator:
+    """Prepared two-site H_eff (ct_heff_create): the environments and MPO tensors
+    are fixed, ``apply(theta)`` is apply_two_site_heff (dmrg.mojo:254-374)
+    without re-deriving W_i W_{i+1} and the precision-specific copies of L and R
+    on every call - the form a Lanczos-type eigensolver uses.  With FP64 the
+    operator reads L and R in place; this object keeps them alive."""
+
+    def __init__(self, L_env: DenseTensor, R_env: DenseTensor, W_i: MPOSite, W_ip1: MPOSite, ctx: DeviceContext,
+                 prec: int | None = None):
+        wL, Dl, Dlb = L_env.shape
+        wR, Dr, Drb = R_env.shape
+        wM = W_i.tensor.shape[3]
+        d = W_i.tensor.shape[1]
+        if W_i.tensor.shape[0] != wL or W_ip1.tensor.shape[3] != wR or W_ip1.tensor.shape[0] != wM \
+                or W_ip1.tensor.shape[1] != d:
+            raise ValueError("HeffOperator: shape mismatch")
+        dt = L_env.dtype
+        for t in (R_env, W_i.tensor, W_ip1.tensor):
+            if t.dtype != dt:
+                raise TypeError("HeffOperator: L, R and W must share one dtype")
+        self.ctx = ctx
+        self.dtype = dt
+        self.dims = (Dl, Dlb, d, Dr, Drb)
+        self._keep = (L_env, R_env, W_i, W_ip1)
+        p = default_prec(dt) if prec is None else prec
+        op = ctypes.c_void_p()
+        call("ct_heff_create", ctx.handle, p, L_env.dtype_code, L_env.ptr(), W_i.tensor.ptr(), W_ip1.tensor.ptr(),
+             R_env.ptr(), Dl, Dlb, d, Dr, Drb, wL, wM, wR, ctypes.byref(op))
+        self.op = op
+
+    def apply(self, theta: DenseTensor, out: torch.Tensor | None = None) -> DenseTensor:
+        Dl, Dlb, d, Dr, Drb = self.dims
+        if list(theta.shape) != [Dl, d, d, Dr] or theta.dtype != self.dtype:
+            raise ValueError("HeffOperator.apply: theta must be [Dl, d, d, Dr] of the operator's dtype")
+        if out is None:
+            out = torch.empty(Dlb * d * d * Drb, dtype=self.dtype, device=theta.storage.device)
+        call("ct_heff_apply_prepared", self.op, theta.ptr(), ctypes.c_void_p(out.data_ptr()))
+        return DenseTensor(out, [Dlb, d, d, Drb])
+
+    def close(self) -> None:
+        if getattr(self, "op", None) is not None and self.op.value:
+            _lib.load().ct_heff_destroy(self.op)
+            self.op = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def lanczos_two_site_optimize(initial_theta: DenseTensor, L_env: DenseTensor, R_env: DenseTensor, W_i: MPOSite,
                               W_ip1: MPOSite, ctx: DeviceContext, max_iter: int, tol: float,
                               prec: int = PREC_FP64, refine: int = 0, return_info: bool = False,

@@ -116,6 +116,22 @@ int ct_heff_apply(ct_handle_t h, int prec, int dtype, const void* L, const void*
                   const void* R, const void* theta, void* out, int64_t Dl, int64_t Dlb, int64_t d, int64_t Dr,
                   int64_t Drb, int64_t wL, int64_t wM, int64_t wR);
 
+/* Prepared H_eff (no reference counterpart: the prepared form of
+ * apply_two_site_heff, dmrg.mojo:254-374, for callers that apply one local
+ * H_eff many times, e.g. their own Lanczos-type eigensolver).  ct_heff_create
+ * builds W2 = Wi.Wj and the precision-specific copies of L and R (FP32 copies
+ * or 3xBF16 split rows) into buffers owned by the operator; with prec FP64 the
+ * operator reads L and R in place, so they must stay valid and unchanged.
+ * ct_heff_apply_prepared(op, theta, out) is ct_heff_apply with those
+ * arguments (theta/out of the operator's dtype), on the handle's stream; the
+ * handle must outlive the operator.  ct_heff_destroy synchronises the stream. */
+typedef struct ct_heff_s* ct_heff_t;
+int ct_heff_create(ct_handle_t h, int prec, int dtype, const void* L, const void* Wi, const void* Wj, const void* R,
+                   int64_t Dl, int64_t Dlb, int64_t d, int64_t Dr, int64_t Drb, int64_t wL, int64_t wM, int64_t wR,
+                   ct_heff_t* op);
+int ct_heff_apply_prepared(ct_heff_t op, const void* theta, void* out);
+int ct_heff_destroy(ct_heff_t op);
+
 /* update_left_environment (src/state/environments.mojo:99-201):
  *   out[wR,Dr,Dr'] = sum L[wL,Dl,Dl'] A[Dl,s,Dr] W[wL,s,s',wR] A[Dl',s',Dr']
  * L [wL,Dl,Dl], A [Dl,d,Dr], W [wL,d,d,wR], out [wR,Dr,Dr]. */

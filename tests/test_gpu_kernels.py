@@ -115,6 +115,30 @@ def test_gemm_bf16x3_nt_tilings(ctx, M, N, K, dt, extra):
     assert re < 2e-5, re
 
 
+@pytest.mark.parametrize("prec", [ct.PREC_FP64, ct.PREC_FP32, ct.PREC_BF16X3])
+def test_heff_prepared_operator_matches_one_shot(ctx, prec):
+    """ct_heff_create / ct_heff_apply_prepared run the same kernels as
+    ct_heff_apply on the same inputs (identical results), and the operator owns
+    its buffers: calls with other shapes on the same handle in between do not
+    disturb it."""
+    rng = np.random.default_rng(21)
+    Dl, Dr, w = 96, 80, 8
+    L = rng.standard_normal((w, Dl, Dl))
+    R = rng.standard_normal((w, Dr, Dr))
+    W = ham.j1j2_tensors(6, 1.0, 0.5)[2]
+    tL, tR, tW = dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, W))
+    op = ct.HeffOperator(tL, tR, tW, tW, ctx, prec=prec)
+    for seed in (1, 2):
+        th = dev(ctx, np.random.default_rng(seed).standard_normal((Dl, 2, 2, Dr)))
+        ref = ct.apply_two_site_heff(th, tL, tR, tW, tW, ctx, prec=prec).to_numpy()
+        # another local problem on the same handle, different shape
+        L2, R2 = dev(ctx, rng.standard_normal((w, 40, 40))), dev(ctx, rng.standard_normal((w, 56, 56)))
+        ct.apply_two_site_heff(dev(ctx, rng.standard_normal((40, 2, 2, 56))), L2, R2, tW, tW, ctx, prec=prec)
+        got = op.apply(th).to_numpy()
+        assert np.array_equal(got, ref)
+    op.close()
+
+
 def test_heff_bf16x3_large_vs_fp64(ctx):
     """3xBF16 H_eff at D=1024, w=8 (J1-J2 bulk MPO): both matvec GEMMs take
     the 256x256 direct-to-LDS tiling (the last one with split-K), against the
