@@ -328,6 +328,50 @@ int ct_theta_split(ct_handle_t h, int svd_mode, const double* theta, int64_t Dl,
     return 0;
 }
 
+int ct_sytrd(ct_handle_t h, int64_t n, double* A, double* d, double* e, double* tau) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(ct::sytrd_coop_supported(n), 2, "n must be in [2, 2048]");
+    void* w;
+    CT_CHECK(ct::ws_get(h, ct::WS_TMP4, 1024 * 8, &w));
+    int* err = (int*)w;
+    CT_HIP(hipMemsetAsync(err, 0, sizeof(int), h->stream));
+    bool ok = false;
+    CT_CHECK(ct::sytrd_coop(h, A, n, d, e, tau, err, &ok));
+    if (!ok) {
+        ct::set_last_error("tridiagonalization grid does not fit on the device");
+        return CT_STATUS_NOT_SUPPORTED;
+    }
+    int herr = 0;
+    CT_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    CT_HIP(hipStreamSynchronize(h->stream));
+    if (herr != 0) {
+        ct::set_last_error("tridiagonalization: grid-wide wait timed out");
+        return CT_STATUS_SVD_FAILED;
+    }
+    return 0;
+}
+
+int ct_syevd(ct_handle_t h, int64_t n, double* A, double* w, double* Z) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(n >= 1, 2, "n must be >= 1");
+    void* ws;
+    CT_CHECK(ct::ws_get(h, ct::WS_TMP4, 1024 * 8, &ws));
+    int* info = (int*)ws;
+    CT_CHECK(ct::syevd(h, A, n, w, Z, info));
+    int hinfo[2] = {0, 0};
+    CT_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, h->stream));
+    CT_HIP(hipStreamSynchronize(h->stream));
+    if (hinfo[1] != 0) {
+        ct::set_last_error("tridiagonalization: grid-wide wait timed out");
+        return CT_STATUS_SVD_FAILED;
+    }
+    if (hinfo[0] != 0) {
+        ct::set_last_error("symmetric eigensolver failed, info=" + std::to_string(hinfo[0]));
+        return CT_STATUS_SVD_FAILED;
+    }
+    return 0;
+}
+
 int ct_heff_adjoint(ct_handle_t h, const double* L, const double* Wi, const double* Wj, const double* R,
                     const double* theta, double* gWi, double* gWj, int64_t Dl, int64_t d, int64_t Dr, int64_t wL,
                     int64_t wM, int64_t wR) {

@@ -2,6 +2,7 @@
 // its own inline copy; the library is built without relocatable device code).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cstdint>
 
 namespace ct {
 
@@ -10,6 +11,31 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
+}
+
+// Sum over the 64 lanes of a wavefront through DPP lane moves (no LDS
+// crossbar): butterfly inside each row of 16 lanes, then the four row sums in
+// a fixed order.  Every lane gets the same value, and every wavefront that
+// sums the same 64 inputs gets the same bits.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double readlane64(double v, int lane) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)u, lane);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), lane);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_mov64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov64<0x141>(v);  // row_half_mirror
+    v += dpp_mov64<0x140>(v);  // row_mirror
+    return (readlane64(v, 0) + readlane64(v, 16)) + (readlane64(v, 32) + readlane64(v, 48));
 }
 
 // Sum over a 256-thread block (4 wavefronts); valid in every thread.

@@ -1,0 +1,404 @@
+// Symmetric eigensolver behind the Gram-matrix two-site split.
+//
+// rocSOLVER's dsyevd spends about 70% of its time in the Householder
+// tridiagonalization (dsytrd), which issues several small kernels per column
+// (about 9700 launches at n = 2048) and is bound by their latency.  Here the
+// tridiagonalization is one cooperative kernel: the matrix stays in the
+// registers of P = ceil(n/R) workgroups (R = 8 or 16 rows each, one
+// workgroup per CU)
+// and each column costs one grid-wide exchange.  The tridiagonal
+// eigenproblem (dstedc) and the back-transformation (dormtr) stay with
+// rocSOLVER: the kernel writes exactly dsytrd's lower-triangle output.
+//
+// Algorithm: the unblocked Householder reduction (LAPACK dsytd2, lower).
+// For k = 0 .. n-2, with x = A[k, k+1:] (= A[k+1:, k]):
+//   (beta, tau, v) = dlarfg(x)          v[k+1] = 1, v[c] = 0 for c <= k
+//   p = tau A22 v,  alpha = -tau/2 (p.v),  w = p + alpha v
+//   A22 -= v w^T + w v^T                 A22 = A[k+1:, k+1:]
+// Distribution: row r lives in workgroup r mod P; thread t of a workgroup of
+// NT threads holds columns t, t+NT, ... of its R rows.  Step k:
+//   1. each workgroup forms p for its rows and publishes it with its share of
+//      p.v; the owner of row k+1 also publishes that row as it stands before
+//      step k;
+//   2. grid-wide arrival: one flag per workgroup, polled by every workgroup;
+//   3. every workgroup reads p and row k+1, applies step k to row k+1 and
+//      forms reflector k+1 itself.  All workgroups run the same arithmetic on
+//      the same data, so every copy of v, tau and w is bitwise identical and
+//      no second exchange is needed.  Then each workgroup updates its rows.
+// Exchange memory is write-once: step k has its own region (one 128-byte line
+// per workgroup with its R p values and its share of p.v, then row k+1),
+// never reused within a launch, and every line is written by one workgroup.
+// Writers store with agent-scope (write-through) stores, each wave waits for
+// its stores to complete, and only then is the arrival flag set (agent-scope
+// store; readers poll the flags with agent-scope loads).  A reader loads a
+// region only after it has seen every flag of that step.  No cache can hold
+// an earlier copy of a region's line (the line was never read before in this
+// launch, and caches are invalidated at kernel start), so plain loads return
+// the published data and the 32 workgroups of an XCD share one L2 fill.  This
+// avoids both the L2 write-back of an agent-scope release fence and the L2
+// invalidation of an acquire fence, which measured 2.0 us per step together
+// at n = 2048.
+#include "ct_internal.h"
+#include "ct_device.h"
+
+namespace ct {
+namespace {
+
+constexpr int TR_MAXN = 2048;
+
+struct TrArgs {
+    double* A;        // [n][lda]: symmetric input; output rows k hold d_k, e_k, v_k (see header comment)
+    int64_t lda;
+    int n;
+    double *d, *e, *tau;
+    double* xb;       // n-1 exchange regions of xs doubles: P lines of LW (p[R], share) | row[np]
+    int64_t xs;
+    unsigned* flags;  // arrival counts, flag g at flags[g * fstride], zero at launch
+    int fstride;
+    int* err;         // set to 1 when an arrival wait exceeds spin_limit polls
+    int spin_limit;
+};
+
+// a - (vr wc + wr vc): one rounding order everywhere, so that the owner of a
+// row and every other workgroup get bitwise the same values.
+__device__ __forceinline__ double upd2(double a, double vr, double wr, double vc, double wc) {
+    return fma(-wr, vc, fma(-vr, wc, a));
+}
+
+// Sum of one value per thread over the NT threads of the workgroup, in a
+// fixed order (every workgroup summing the same inputs gets the same bits).
+template <int NT>
+__device__ __forceinline__ double block_sum_fixed(double v, double* red) {
+    constexpr int W = NT / 64;
+    v = wave_sum_dpp(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = red[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) s += red[w];
+    return s;
+}
+
+// LAPACK dlarfg on the row slice x[c], c >= k+1 (alpha = x[k+1]): v[k+1] = 1,
+// v[c] = x[c] / (alpha - beta) for c >= k+2, zero elsewhere.  Every thread of
+// every workgroup gets the same tau, beta, v.
+template <int NT, int NC>
+__device__ __forceinline__ void make_reflector(int k, int n, const int (&c)[NC], const double (&x)[NC],
+                                               double (&v)[NC], double& tau, double& beta, double* red,
+                                               double* bcast) {
+    double s2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        if (c[j] >= k + 2 && c[j] < n) s2 = fma(x[j], x[j], s2);
+        if (c[j] == k + 1) *bcast = x[j];
+    }
+    s2 = block_sum_fixed<NT>(s2, red);
+    const double alpha = *bcast;
+    if (s2 == 0.0) {
+        tau = 0.0;
+        beta = alpha;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) v[j] = c[j] == k + 1 ? 1.0 : 0.0;
+    } else {
+        const double nrm = sqrt(fma(alpha, alpha, s2));
+        beta = alpha >= 0.0 ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        const double scal = 1.0 / (alpha - beta);
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+            v[j] = c[j] == k + 1 ? 1.0 : ((c[j] >= k + 2 && c[j] < n) ? x[j] * scal : 0.0);
+    }
+}
+
+// NT threads per workgroup, R matrix rows per workgroup, NC columns per thread
+// (n <= NT*NC, P*R >= n).
+template <int NT, int R, int NC>
+__global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
+    constexpr int W = NT / 64;
+    constexpr int LW = R + 1 <= 16 ? 16 : 32;  // doubles per exchange line (p[R], share)
+    const int n = q.n, P = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    __shared__ double red[W][R];
+    __shared__ double redw[W];
+    __shared__ double bcast;
+    __shared__ double ploc[R], vr[2][R], wr[R];
+    __shared__ int dead;
+
+    // Column c[j] of this thread; where its p value sits in a step's
+    // exchange region; which of this workgroup's rows it is (-1: none).
+    int c[NC], poff[NC], lrow[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        c[j] = t + NT * j;
+        poff[j] = LW * (c[j] % P) + c[j] / P;
+        const int dl = c[j] - g;
+        lrow[j] = (c[j] < n && dl >= 0 && dl % P == 0 && dl / P < R) ? dl / P : -1;
+    }
+    double a[R][NC];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int r = g + P * s;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) a[s][j] = (r < n && c[j] < n) ? q.A[(int64_t)r * q.lda + c[j]] : 0.0;
+    }
+    if (t < R) { vr[0][t] = 0.0; vr[1][t] = 0.0; }
+    if (t == 0) dead = 0;
+
+    // reflector 0 from row 0 of the input
+    double v[NC], x[NC], w[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x[j] = c[j] < n ? q.A[c[j]] : 0.0;
+    double tau = 0.0, beta = 0.0;
+    if (n >= 2) {
+        make_reflector<NT, NC>(0, n, c, x, v, tau, beta, redw, &bcast);
+    } else {
+#pragma unroll
+        for (int j = 0; j < NC; ++j) v[j] = 0.0;
+    }
+    if (g == 0) {
+        if (t == 0) {
+            q.d[0] = x[0];
+            if (n >= 2) { q.e[0] = beta; q.tau[0] = tau; }
+        }
+        // row 0 (slot 0 of workgroup 0) in output form
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+            if (c[j] >= 1 && c[j] < n) a[0][j] = c[j] == 1 ? beta : v[j];
+    }
+    int cur = 0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j)
+        if (lrow[j] >= 0) vr[0][lrow[j]] = v[j];
+    __syncthreads();
+
+    int own1 = 1 % P, s1 = 1 / P;  // owner and slot of row i+1
+    for (int i = 0; i + 1 < n; ++i) {
+        double* xl = q.xb + (int64_t)i * q.xs;  // line g: p at rows g + P*s (s < R), share of p.v
+        double* xr = xl + LW * P;               // row i+1 before step i
+
+        // ---- 1. p at this workgroup's rows, published
+        double ps[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < NC; ++j) acc = fma(a[s][j], v[j], acc);
+            ps[s] = wave_sum_dpp(acc);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int s = 0; s < R; ++s) red[wave][s] = ps[s];
+        }
+        if (g == own1) {
+#pragma unroll
+            for (int s = 0; s < R; ++s)
+                if (s == s1) {
+#pragma unroll
+                    for (int j = 0; j < NC; ++j)
+                        if (c[j] > i && c[j] < n)
+                            __hip_atomic_store(&xr[c[j]], a[s][j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double share = 0.0;
+            if (lane < R) {
+                const int r = g + P * lane;
+                double pv = red[0][lane];
+#pragma unroll
+                for (int ww = 1; ww < W; ++ww) pv += red[ww][lane];
+                pv = (r > i && r < n) ? tau * pv : 0.0;
+                __hip_atomic_store(&xl[LW * g + lane], pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ploc[lane] = pv;
+                share = pv * vr[cur][lane];
+            }
+            share = wave_sum_dpp(share);
+            if (lane == 0) __hip_atomic_store(&xl[LW * g + R], share, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // Every wave waits until its own exchange stores have completed (the
+        // barrier alone orders only LDS traffic), then the flag is set.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (t == 0)
+            __hip_atomic_store(&q.flags[g * q.fstride], (unsigned)(i + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+
+        // ---- 2. wait for every workgroup's step-i data
+        if (wave == 0 && !dead) {
+            const unsigned target = (unsigned)(i + 1);
+            int polls = 0;
+            while (true) {
+                bool ok = true;
+                for (int m = lane; m < P; m += 64)
+                    ok = ok && __hip_atomic_load(&q.flags[m * q.fstride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+                                   target;
+                if (__all(ok)) break;
+                if (++polls > q.spin_limit) {
+                    if (lane == 0) {
+                        dead = 1;
+                        atomicExch(q.err, 1);
+                    }
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        // loads below stay after the poll (write-once lines: no cache invalidation needed)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        __syncthreads();
+
+        // ---- 3. w, row i+1 after step i, reflector i+1, local update
+        double pdotv = 0.0;  // same summation order in every wave of every workgroup
+        for (int m = lane; m < P; m += 64) pdotv += xl[LW * m + R];
+        double pc[NC], rc[NC];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            pc[j] = (c[j] > i && c[j] < n) ? xl[poff[j]] : 0.0;
+            rc[j] = (c[j] > i && c[j] < n) ? xr[c[j]] : 0.0;
+        }
+        const double p1 = xl[LW * own1 + s1];
+        pdotv = wave_sum_dpp(pdotv);
+        const double alpha = -0.5 * tau * pdotv;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) w[j] = fma(alpha, v[j], pc[j]);
+        const double w1 = fma(alpha, 1.0, p1);
+        if (t < R) wr[t] = fma(alpha, vr[cur][t], ploc[t]);
+#pragma unroll
+        for (int j = 0; j < NC; ++j) x[j] = (c[j] > i && c[j] < n) ? upd2(rc[j], 1.0, w1, v[j], w[j]) : 0.0;
+        if (g == 0) {
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+                if (c[j] == i + 1) q.d[i + 1] = x[j];
+        }
+        double vn[NC];
+        double taun = 0.0, betan = 0.0;
+        if (i + 2 < n) {
+            make_reflector<NT, NC>(i + 1, n, c, x, vn, taun, betan, redw, &bcast);
+            if (g == 0 && t == 0) { q.e[i + 1] = betan; q.tau[i + 1] = taun; }
+        } else {
+#pragma unroll
+            for (int j = 0; j < NC; ++j) vn[j] = 0.0;
+            __syncthreads();
+        }
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            if (g == own1 && s == s1) {
+#pragma unroll
+                for (int j = 0; j < NC; ++j)
+                    if (c[j] > i && c[j] < n) a[s][j] = c[j] == i + 1 ? x[j] : (c[j] == i + 2 ? betan : vn[j]);
+            } else {
+                const double vs = vr[cur][s], ws = wr[s];
+#pragma unroll
+                for (int j = 0; j < NC; ++j) a[s][j] = upd2(a[s][j], vs, ws, v[j], w[j]);
+            }
+        }
+        const int nxt = cur ^ 1;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            if (lrow[j] >= 0) vr[nxt][lrow[j]] = vn[j];
+            v[j] = vn[j];
+        }
+        tau = taun;
+        cur = nxt;
+        if (++own1 == P) { own1 = 0; ++s1; }
+        __syncthreads();
+    }
+
+    // rows back in dsytrd's lower format (column-major view): A[r][c], c >= r
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int r = g + P * s;
+        if (r < n) {
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+                if (c[j] >= r && c[j] < n) q.A[(int64_t)r * q.lda + c[j]] = a[s][j];
+        }
+    }
+}
+
+__global__ void zero_u32_kernel(unsigned* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+template <int NT, int R, int NC>
+int launch_sytrd(Handle* h, TrArgs a, bool* ok) {
+    constexpr int LW = R + 1 <= 16 ? 16 : 32;
+    const int n = a.n, P = (n + R - 1) / R;
+    static int per_cu = -1;  // co-resident workgroups per CU (same for every MI355X)
+    if (per_cu < 0) CT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sytrd_coop_kernel<NT, R, NC>, NT, 0));
+    if ((int64_t)per_cu * h->num_cu < P) {
+        *ok = false;
+        return 0;
+    }
+    const int64_t np = (n + 63) / 64 * 64;
+    a.xs = (int64_t)LW * P + np;  // multiple of 16 doubles: every line 128-byte aligned
+    void* ws;
+    CT_CHECK(ws_get(h, WS_EIG, (size_t)(n * a.xs) * 8 + (size_t)P * a.fstride * 4, &ws));
+    a.xb = (double*)ws;
+    a.flags = (unsigned*)((double*)ws + n * a.xs);
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(P), dim3(64), 0, h->stream, a.flags, P * a.fstride);
+    CT_HIP(hipGetLastError());
+    void* kargs[] = {&a};
+    CT_HIP(hipLaunchCooperativeKernel((const void*)sytrd_coop_kernel<NT, R, NC>, dim3(P), dim3(NT), kargs, 0,
+                                      h->stream));
+    *ok = true;
+    return 0;
+}
+
+}  // namespace
+
+bool sytrd_coop_supported(int64_t n) { return n >= 2 && n <= TR_MAXN; }
+
+int sytrd_coop(Handle* h, double* A, int64_t n, double* d, double* e, double* tau, int* d_err, bool* ok) {
+    *ok = false;
+    if (!sytrd_coop_supported(n)) return 0;
+    TrArgs a;
+    a.A = A; a.lda = n; a.n = (int)n;
+    a.d = d; a.e = e; a.tau = tau;
+    a.err = d_err;
+    a.spin_limit = 1 << 22;
+    a.fstride = 16;  // one flag per 64 bytes: measured 2x shorter arrival waits than packed flags
+    // 8 rows per workgroup (64 doubles of the matrix per thread at n = 2048);
+    // measured faster than 16-row workgroups of 512 threads, whose per-step
+    // reductions cost more than the smaller grid saves
+    if (n <= 256) return launch_sytrd<256, 8, 1>(h, a, ok);
+    if (n <= 512) return launch_sytrd<256, 8, 2>(h, a, ok);
+    if (n <= 1024) return launch_sytrd<256, 8, 4>(h, a, ok);
+    return launch_sytrd<256, 8, 8>(h, a, ok);
+}
+
+// Symmetric eigendecomposition: A (n x n, symmetric, destroyed) -> ascending
+// eigenvalues w[n], eigenvector j in row j of Z (LAPACK's column-major
+// eigenvector matrix read row-major).  Device status words: info[0] from
+// dstedc / dsyevd, info[1] set when the tridiagonalization's arrival wait
+// timed out.  The caller reads them after its next synchronisation.
+int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info) {
+    void* wsb;
+    CT_CHECK(ws_get(h, WS_EIG2, (size_t)(2 * n + 8) * 8, &wsb));
+    double* e = (double*)wsb;
+    double* tau = e + n;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, h->stream, (unsigned*)info, 2);
+    CT_HIP(hipGetLastError());
+    bool ok = false;
+    CT_CHECK(sytrd_coop(h, A, n, w, e, tau, info + 1, &ok));
+    rocblas_status st;
+    if (ok) {
+        st = rocsolver_dstedc(h->blas, rocblas_evect_tridiagonal, (rocblas_int)n, w, e, Z, (rocblas_int)n,
+                              (rocblas_int*)info);
+        if (st == rocblas_status_success)
+            st = rocsolver_dormtr(h->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                  (rocblas_int)n, (rocblas_int)n, A, (rocblas_int)n, tau, Z, (rocblas_int)n);
+    } else {
+        CT_HIP(hipMemcpyAsync(Z, A, (size_t)(n * n) * 8, hipMemcpyDeviceToDevice, h->stream));
+        st = rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, Z, (rocblas_int)n,
+                              w, e, (rocblas_int*)info);
+    }
+    if (st != rocblas_status_success) {
+        set_last_error(std::string("rocSOLVER eigensolver: ") + rocblas_status_to_string(st));
+        return CT_STATUS_ROCBLAS_ERROR_BASE + (int)st;
+    }
+    return 0;
+}
+
+}  // namespace ct

@@ -102,7 +102,9 @@ enum WsSlot {
     WS_SPLITK = 24,  // split-K partial sums of the 3xBF16 GEMM
     WS_W2PAD64 = 25, // zero-padded fused MPO coefficients, FP64 (FP64 H_eff)
     WS_W2PAD32 = 26, // zero-padded fused MPO coefficients, FP32 (FP32 / 3xBF16 H_eff)
-    WS_COUNT = 27
+    WS_EIG = 27,     // tridiagonalization exchange buffers and arrival flags
+    WS_EIG2 = 28,    // tridiagonal off-diagonal and reflector scalars
+    WS_COUNT = 29
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);
@@ -130,8 +132,6 @@ struct GemmDesc {
 int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int tC, void* C,
          const GemmDesc& g);
 
-// 3xBF16 operands pre-split into hi/lo BF16 planes with k-contiguous rows:
-// element (row, k) of batch b at hi[b*bs + row*ld + k] (same for lo).
 // 3xBF16 operand in "split rows": row r of batch b starts at p + b*bs + r*ld;
 // for each 32-wide block kb of the contraction index it holds 32 hi = bf16(x)
 // values followed by the 32 lo = bf16(x - hi) values (one 128-byte line), with
@@ -267,6 +267,21 @@ int svd_split_gram(Handle* h, const double* theta, int64_t m, int64_t n, int64_t
                    double* Ai, double* Aj, SvdResult* r);
 int svd_trunc(Handle* h, const double* A, int64_t m, int64_t n, int64_t chi_max, double eps, double* U,
               double* S, double* Vt, SvdResult* r);
+
+// Symmetric eigensolver (ct_eig.hip) --------------------------------------
+// Householder tridiagonalization of the symmetric n x n FP64 matrix A (row
+// stride n, both triangles) in one cooperative kernel, output in LAPACK
+// dsytrd's lower format (column-major view): d[n], e[n-1], tau[n-1], row k of
+// A holds v_k at columns k+2.. .  *ok = false (nothing launched) when n is
+// outside [2, 2048] or the grid cannot be co-resident.  *d_err (device) is
+// set to 1 if a grid-wide wait timed out.
+bool sytrd_coop_supported(int64_t n);
+int sytrd_coop(Handle* h, double* A, int64_t n, double* d, double* e, double* tau, int* d_err, bool* ok);
+// Eigendecomposition of the symmetric n x n matrix A (destroyed): ascending
+// eigenvalues w[n]; eigenvector j in row j of Z[n][n].  info[0] (device):
+// rocSOLVER info; info[1]: tridiagonalization wait timeout.  Both are read by
+// the caller after its next stream synchronisation.
+int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info);
 
 }  // namespace ct
 

@@ -160,7 +160,7 @@ int svd_split_gram(Handle* h, const double* theta, int64_t m, int64_t n, int64_t
     double* G = (double*)wg;
     double* lam = G + k * k;
     double* e = lam + k;
-    rocblas_int* info = (rocblas_int*)(e + k + 8);
+    int* info = (int*)(e + k + 8);
     double* part = (double*)wp;
     (void)wi;
     hipLaunchKernelGGL(sumsq_partial, dim3(512), dim3(256), 0, h->stream, theta, m * n, part);
@@ -179,21 +179,23 @@ int svd_split_gram(Handle* h, const double* theta, int64_t m, int64_t n, int64_t
         }
         CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, theta, CT_F64, theta, CT_F64, G, g));
     }
-    rocblas_status st = rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)k, G,
-                                         (rocblas_int)k, lam, e, info);
-    if (st != rocblas_status_success) {
-        set_last_error(std::string("rocsolver_dsyevd: ") + rocblas_status_to_string(st));
-        return CT_STATUS_ROCBLAS_ERROR_BASE + (int)st;
-    }
+    void* wz;
+    CT_CHECK(ws_get(h, WS_SVD3, (size_t)(k * k) * 8, &wz));
+    double* Z = (double*)wz;  // eigenvector j in row j
+    CT_CHECK(syevd(h, G, k, lam, Z, info));
     double* hp = h->h_pinned;
     CT_HIP(hipMemcpyAsync(hp, lam, (size_t)k * 8, hipMemcpyDeviceToHost, h->stream));
     CT_HIP(hipMemcpyAsync(hp + k, part + 600, 8, hipMemcpyDeviceToHost, h->stream));
-    CT_HIP(hipMemcpyAsync(hp + k + 1, info, sizeof(rocblas_int), hipMemcpyDeviceToHost, h->stream));
+    CT_HIP(hipMemcpyAsync(hp + k + 1, info, 2 * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     CT_HIP(hipStreamSynchronize(h->stream));
-    int hinfo = 0;
-    memcpy(&hinfo, hp + k + 1, sizeof(int));
-    if (hinfo != 0) {
-        set_last_error("syevd failed, info=" + std::to_string(hinfo));
+    int hinfo[2] = {0, 0};
+    memcpy(hinfo, hp + k + 1, sizeof(hinfo));
+    if (hinfo[1] != 0) {
+        set_last_error("tridiagonalization: grid-wide wait timed out");
+        return CT_STATUS_SVD_FAILED;
+    }
+    if (hinfo[0] != 0) {
+        set_last_error("symmetric eigensolver failed, info=" + std::to_string(hinfo[0]));
         return CT_STATUS_SVD_FAILED;
     }
     const double norm_sq = hp[k];
@@ -228,7 +230,7 @@ int svd_split_gram(Handle* h, const double* theta, int64_t m, int64_t n, int64_t
     r->discarded = norm_sq > 0.0 ? (norm_sq - kept) / norm_sq : 0.0;
     if (chi == 0) return 0;
     double* sel = (double*)wsel;  // [chi][k], descending
-    hipLaunchKernelGGL(take_top_rows_kernel, dim3(1024), dim3(256), 0, h->stream, (const double*)G, sel, k, chi);
+    hipLaunchKernelGGL(take_top_rows_kernel, dim3(1024), dim3(256), 0, h->stream, (const double*)Z, sel, k, chi);
     CT_HIP(hipGetLastError());
     if (l2r) {
         const int64_t shape[2] = {chi, m};

@@ -186,6 +186,20 @@ int ct_theta_split(ct_handle_t h, int svd_mode, const double* theta, int64_t Dl,
                    int64_t chi_max, double eps, int left_to_right, double* A_i, double* A_j, int64_t* chi,
                    double* discarded_weight);
 
+/* Symmetric eigensolver of the CT_SVD_GRAM split (no reference counterpart:
+ * the reference's SVD is LAPACKE dgesdd through native/svd_shim.c:11-23).
+ * ct_sytrd: Householder tridiagonalization of the symmetric n x n FP64 matrix
+ * A (device, row stride n, both triangles; 2 <= n <= 2048) in one cooperative
+ * kernel.  Output exactly as LAPACK dsytrd with uplo = 'L' on the column-major
+ * view of A: d[n], e[n-1], tau[n-1]; row k of A holds d_k at column k, e_k at
+ * column k+1 and the reflector v_k at columns k+2..n-1.  Synchronises.
+ * ct_syevd: A (destroyed) -> ascending eigenvalues w[n] and eigenvector j in
+ * row j of Z[n][n] (LAPACK's column-major eigenvector matrix); ct_sytrd +
+ * rocSOLVER dstedc + dormtr for n <= 2048, rocSOLVER dsyevd above.
+ * Synchronises. */
+int ct_sytrd(ct_handle_t h, int64_t n, double* A, double* d, double* e, double* tau);
+int ct_syevd(ct_handle_t h, int64_t n, double* A, double* w, double* Z);
+
 /* ---- sweep driver ------------------------------------------------------
  * dmrg_two_site (src/algorithms/dmrg.mojo:377-483) with DMRGWorkspace
  * (:39-90) and dmrg_local_update_two_site (:759-844).  The MPS and MPO are
