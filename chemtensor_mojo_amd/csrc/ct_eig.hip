@@ -321,6 +321,58 @@ __global__ void zero_u32_kernel(unsigned* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+// ------------------------------------------------- back-transformation
+// Eigenvectors of A from those of T: u = Q z with Q = H_0 H_1 ... H_{n-2}.
+// With eigenvectors as rows (E = [z_j^T]), E' = E Q^T = E H_{n-2} ... H_0.
+// Blocks of BT_B reflectors: H_k0 ... H_{k0+B-1} = I - V T V^T (LAPACK dlarft,
+// forward, columnwise; V = Vt_m^T), so a block applies as
+//   E <- E - (E Vt_m^T) (T^T Vt_m),
+// last block first.  All Vt_m, T_m and M_m = T_m^T Vt_m are built up front
+// (batched GEMMs and one small kernel); the sequential part is two GEMMs per
+// block on the rows actually needed (the chi kept eigenvectors).
+constexpr int BT_B = 64;
+
+// Vt[k][c] = v_k[c]: 0 for c <= k, 1 at c = k+1, A[k][c] beyond; rows k >= n-1
+// (padding up to a whole block) are zero with tau 0.
+__global__ void build_vt_kernel(const double* __restrict__ A, const double* __restrict__ tau, double* __restrict__ Vt,
+                                double* __restrict__ taup, int n, int nr) {
+    const int64_t total = (int64_t)nr * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int k = (int)(t / n), c = (int)(t - (int64_t)k * n);
+        double v = 0.0;
+        if (k < n - 1) v = c <= k ? 0.0 : (c == k + 1 ? 1.0 : A[(int64_t)k * n + c]);
+        Vt[t] = v;
+        if (c == 0) taup[k] = k < n - 1 ? tau[k] : 0.0;
+    }
+}
+
+// T (upper triangular) of one block from S = V^T V: T(i,i) = tau_i,
+// T(0:i, i) = -tau_i T(0:i, 0:i) S(0:i, i).  One workgroup per block.
+__global__ __launch_bounds__(256) void larft_kernel(const double* __restrict__ S, const double* __restrict__ taup,
+                                                    double* __restrict__ T) {
+    constexpr int B = BT_B;
+    __shared__ double s[B][B + 1], tt[B][B + 1];
+    const int m = blockIdx.x, r = threadIdx.x;
+    const double* Sm = S + (int64_t)m * B * B;
+    for (int t = r; t < B * B; t += blockDim.x) {
+        s[t / B][t % B] = Sm[t];
+        tt[t / B][t % B] = 0.0;
+    }
+    __syncthreads();
+    for (int i = 0; i < B; ++i) {
+        const double ti = taup[m * B + i];
+        if (r < i) {  // reads columns < i of tt, writes column i
+            double acc = 0.0;
+            for (int q = r; q < i; ++q) acc = fma(tt[r][q], s[q][i], acc);
+            tt[r][i] = -ti * acc;
+        }
+        if (r == i) tt[i][i] = ti;
+        __syncthreads();
+    }
+    double* Tm = T + (int64_t)m * B * B;
+    for (int t = r; t < B * B; t += blockDim.x) Tm[t] = tt[t / B][t % B];
+}
+
 template <int NT, int R, int NC>
 int launch_sytrd(Handle* h, TrArgs a, bool* ok) {
     constexpr int LW = R + 1 <= 16 ? 16 : 32;
@@ -368,12 +420,58 @@ int sytrd_coop(Handle* h, double* A, int64_t n, double* d, double* e, double* ta
     return launch_sytrd<256, 8, 8>(h, a, ok);
 }
 
-// Symmetric eigendecomposition: A (n x n, symmetric, destroyed) -> ascending
-// eigenvalues w[n], eigenvector j in row j of Z (LAPACK's column-major
-// eigenvector matrix read row-major).  Device status words: info[0] from
-// dstedc / dsyevd, info[1] set when the tridiagonalization's arrival wait
-// timed out.  The caller reads them after its next synchronisation.
-int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info) {
+namespace {
+
+int bt_blocks(int64_t n) { return n >= 2 ? (int)((n - 1 + BT_B - 1) / BT_B) : 0; }
+
+// Builds Vt, T and M = T^T Vt for every block (see the back-transformation notes above).
+int bt_prepare(Handle* h, const double* A, const double* tau, int64_t n) {
+    const int nb = bt_blocks(n);
+    if (nb == 0) return 0;
+    const int64_t nr = (int64_t)nb * BT_B;
+    void *wv, *wm, *wst;
+    CT_CHECK(ws_get(h, WS_EIG3, (size_t)(nr * n) * 8, &wv));
+    CT_CHECK(ws_get(h, WS_EIG4, (size_t)(nr * n) * 8, &wm));
+    CT_CHECK(ws_get(h, WS_EIG5, (size_t)(2 * nb * BT_B * BT_B + nr) * 8, &wst));
+    double* Vt = (double*)wv;
+    double* M = (double*)wm;
+    double* S = (double*)wst;
+    double* T = S + (int64_t)nb * BT_B * BT_B;
+    double* taup = T + (int64_t)nb * BT_B * BT_B;
+    hipLaunchKernelGGL(build_vt_kernel, dim3(1024), dim3(256), 0, h->stream, A, tau, Vt, taup, (int)n, (int)nr);
+    CT_HIP(hipGetLastError());
+    {   // S_m = Vt_m Vt_m^T
+        GemmDesc g;
+        g.M = BT_B; g.N = BT_B; g.K = (int)n; g.batch = nb;
+        g.a_kc = 1; g.sAm = n; g.sAk = 1; g.bsA = BT_B * n;
+        g.b_kc = 1; g.sBn = n; g.sBk = 1; g.bsB = BT_B * n;
+        g.ldc = BT_B; g.bsC = BT_B * BT_B;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, Vt, CT_F64, Vt, CT_F64, S, g));
+    }
+    hipLaunchKernelGGL(larft_kernel, dim3(nb), dim3(256), 0, h->stream, (const double*)S, (const double*)taup, T);
+    CT_HIP(hipGetLastError());
+    {   // M_m = T_m^T Vt_m
+        GemmDesc g;
+        g.M = BT_B; g.N = (int)n; g.K = BT_B; g.batch = nb;
+        g.a_kc = 0; g.sAm = 1; g.sAk = BT_B; g.bsA = BT_B * BT_B;
+        g.b_kc = 0; g.sBk = n; g.sBn = 1; g.bsB = BT_B * n;
+        g.ldc = n; g.bsC = BT_B * n;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, T, CT_F64, Vt, CT_F64, M, g));
+    }
+    return 0;
+}
+
+}  // namespace
+
+// Symmetric eigendecomposition in two stages.  Stage 1: A (n x n, symmetric,
+// destroyed) -> ascending eigenvalues w[n] and, in row j of Z[n][n], the
+// eigenvector j of the tridiagonal matrix (LAPACK's column-major eigenvector
+// matrix read row-major); it also builds the back-transformation factors.
+// Device status words: info[0] from dstedc / dsyevd, info[1] set when the
+// tridiagonalization's arrival wait timed out.  Above n = 2048 (or when the
+// cooperative grid cannot be resident) it runs rocSOLVER dsyevd instead and
+// sets *staged = false: Z then already holds the eigenvectors of A.
+int syevd_stage1(Handle* h, double* A, int64_t n, double* w, double* Z, int* info, bool* staged) {
     void* wsb;
     CT_CHECK(ws_get(h, WS_EIG2, (size_t)(2 * n + 8) * 8, &wsb));
     double* e = (double*)wsb;
@@ -386,9 +484,7 @@ int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info) {
     if (ok) {
         st = rocsolver_dstedc(h->blas, rocblas_evect_tridiagonal, (rocblas_int)n, w, e, Z, (rocblas_int)n,
                               (rocblas_int*)info);
-        if (st == rocblas_status_success)
-            st = rocsolver_dormtr(h->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                                  (rocblas_int)n, (rocblas_int)n, A, (rocblas_int)n, tau, Z, (rocblas_int)n);
+        if (st == rocblas_status_success) CT_CHECK(bt_prepare(h, A, tau, n));
     } else {
         CT_HIP(hipMemcpyAsync(Z, A, (size_t)(n * n) * 8, hipMemcpyDeviceToDevice, h->stream));
         st = rocsolver_dsyevd(h->blas, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, Z, (rocblas_int)n,
@@ -398,6 +494,50 @@ int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info) {
         set_last_error(std::string("rocSOLVER eigensolver: ") + rocblas_status_to_string(st));
         return CT_STATUS_ROCBLAS_ERROR_BASE + (int)st;
     }
+    *staged = ok;
+    return 0;
+}
+
+// Stage 2: rows n-rows .. n-1 of Z (the eigenvectors of the `rows` largest
+// eigenvalues) become eigenvectors of A.  Only after a staged stage 1 on the
+// same handle with no other eigensolver call in between.
+int syevd_stage2(Handle* h, int64_t n, double* Z, int64_t rows) {
+    const int nb = bt_blocks(n);
+    if (nb == 0 || rows <= 0) return 0;
+    void *wv, *wm, *wst, *ww;
+    CT_CHECK(ws_get(h, WS_EIG3, 0, &wv));
+    CT_CHECK(ws_get(h, WS_EIG4, 0, &wm));
+    CT_CHECK(ws_get(h, WS_EIG5, 0, &wst));
+    CT_CHECK(ws_get(h, WS_TMP2, (size_t)(rows * BT_B) * 8, &ww));
+    const double* Vt = (const double*)wv;
+    const double* M = (const double*)wm;
+    double* W = (double*)ww;
+    double* E = Z + (n - rows) * n;
+    const double one = 1.0, zero = 0.0, minus_one = -1.0;
+    for (int m = nb - 1; m >= 0; --m) {
+        const int64_t off = (int64_t)m * BT_B + 1;  // Vt_m is zero in columns < off
+        const int64_t K = n - off;
+        // W^T (B x rows) = Vt_m[:, off:] E[:, off:]^T   (column-major views)
+        rocblas_status st = rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, BT_B,
+                                          (rocblas_int)rows, (rocblas_int)K, &one, Vt + (int64_t)m * BT_B * n + off,
+                                          (rocblas_int)n, E + off, (rocblas_int)n, &zero, W, BT_B);
+        // E[:, off:] -= W M_m[:, off:]
+        if (st == rocblas_status_success)
+            st = rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)K,
+                               (rocblas_int)rows, BT_B, &minus_one, M + (int64_t)m * BT_B * n + off, (rocblas_int)n, W,
+                               BT_B, &one, E + off, (rocblas_int)n);
+        if (st != rocblas_status_success) {
+            set_last_error(std::string("rocblas_dgemm (back-transformation): ") + rocblas_status_to_string(st));
+            return CT_STATUS_ROCBLAS_ERROR_BASE + (int)st;
+        }
+    }
+    return 0;
+}
+
+int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info) {
+    bool staged = false;
+    CT_CHECK(syevd_stage1(h, A, n, w, Z, info, &staged));
+    if (staged) CT_CHECK(syevd_stage2(h, n, Z, n));
     return 0;
 }
 

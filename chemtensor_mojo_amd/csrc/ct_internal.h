@@ -104,7 +104,10 @@ enum WsSlot {
     WS_W2PAD32 = 26, // zero-padded fused MPO coefficients, FP32 (FP32 / 3xBF16 H_eff)
     WS_EIG = 27,     // tridiagonalization exchange buffers and arrival flags
     WS_EIG2 = 28,    // tridiagonal off-diagonal and reflector scalars
-    WS_COUNT = 29
+    WS_EIG3 = 29,    // back-transformation: reflector rows Vt
+    WS_EIG4 = 30,    // back-transformation: T^T Vt per block
+    WS_EIG5 = 31,    // back-transformation: V^T V and T per block
+    WS_COUNT = 32
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);
@@ -282,6 +285,12 @@ int sytrd_coop(Handle* h, double* A, int64_t n, double* d, double* e, double* ta
 // rocSOLVER info; info[1]: tridiagonalization wait timeout.  Both are read by
 // the caller after its next stream synchronisation.
 int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info);
+// The same in two stages, so that a caller can read the eigenvalues first and
+// back-transform only the eigenvectors it keeps: stage 1 leaves the
+// eigenvectors of the tridiagonal matrix in Z (final ones when *staged is
+// false); stage 2 turns rows n-rows..n-1 of Z into eigenvectors of A.
+int syevd_stage1(Handle* h, double* A, int64_t n, double* w, double* Z, int* info, bool* staged);
+int syevd_stage2(Handle* h, int64_t n, double* Z, int64_t rows);
 
 }  // namespace ct
 

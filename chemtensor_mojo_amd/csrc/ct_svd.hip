@@ -182,7 +182,8 @@ int svd_split_gram(Handle* h, const double* theta, int64_t m, int64_t n, int64_t
     void* wz;
     CT_CHECK(ws_get(h, WS_SVD3, (size_t)(k * k) * 8, &wz));
     double* Z = (double*)wz;  // eigenvector j in row j
-    CT_CHECK(syevd(h, G, k, lam, Z, info));
+    bool staged = false;
+    CT_CHECK(syevd_stage1(h, G, k, lam, Z, info, &staged));
     double* hp = h->h_pinned;
     CT_HIP(hipMemcpyAsync(hp, lam, (size_t)k * 8, hipMemcpyDeviceToHost, h->stream));
     CT_HIP(hipMemcpyAsync(hp + k, part + 600, 8, hipMemcpyDeviceToHost, h->stream));
@@ -229,6 +230,7 @@ int svd_split_gram(Handle* h, const double* theta, int64_t m, int64_t n, int64_t
     for (int64_t i = 0; i < chi; ++i) kept += sig[i] * sig[i];
     r->discarded = norm_sq > 0.0 ? (norm_sq - kept) / norm_sq : 0.0;
     if (chi == 0) return 0;
+    if (staged) CT_CHECK(syevd_stage2(h, k, Z, chi));  // only the kept eigenvectors
     double* sel = (double*)wsel;  // [chi][k], descending
     hipLaunchKernelGGL(take_top_rows_kernel, dim3(1024), dim3(256), 0, h->stream, (const double*)Z, sel, k, chi);
     CT_HIP(hipGetLastError());
