@@ -118,10 +118,10 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
     constexpr int LW = R + 1 <= 16 ? 16 : 32;  // doubles per exchange line (p[R], share)
     const int n = q.n, P = gridDim.x, g = blockIdx.x, t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    __shared__ double red[W][R];
+    __shared__ double red[W][R + 2];
     __shared__ double redw[W];
     __shared__ double bcast;
-    __shared__ double ploc[R], vr[2][R], wr[R];
+    __shared__ double ploc[R], vr[2][R], vrp[R], wrp[R];
     __shared__ int dead;
 
     // Column c[j] of this thread; where its p value sits in a step's
@@ -141,11 +141,19 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
 #pragma unroll
         for (int j = 0; j < NC; ++j) a[s][j] = (r < n && c[j] < n) ? q.A[(int64_t)r * q.lda + c[j]] : 0.0;
     }
-    if (t < R) { vr[0][t] = 0.0; vr[1][t] = 0.0; }
+    if (t < R) { vr[0][t] = 0.0; vr[1][t] = 0.0; vrp[t] = 0.0; wrp[t] = 0.0; }
     if (t == 0) dead = 0;
 
     // reflector 0 from row 0 of the input
     double v[NC], x[NC], w[NC];
+    // The rank-2 update of step i is applied to the registers during step i+1,
+    // after the arrival flag is set, so it overlaps the wait instead of
+    // delaying the next p: "pending" holds (v, w) of that update at this
+    // thread's columns (vp, wp) and at this workgroup's rows (vrp, wrp; zero
+    // for rows that are finished, so those stay untouched).
+    double vp[NC], wp[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) { vp[j] = 0.0; wp[j] = 0.0; }
 #pragma unroll
     for (int j = 0; j < NC; ++j) x[j] = c[j] < n ? q.A[c[j]] : 0.0;
     double tau = 0.0, beta = 0.0;
@@ -176,8 +184,10 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
         double* xl = q.xb + (int64_t)i * q.xs;  // line g: p at rows g + P*s (s < R), share of p.v
         double* xr = xl + LW * P;               // row i+1 before step i
 
-        // ---- 1. p at this workgroup's rows, published
-        double ps[R];
+        // ---- 1. p_i = tau A^(i) v at this workgroup's rows, published.  The
+        // registers hold A^(i-1) (step i-1 pending), so
+        //   A^(i) v = A^(i-1) v - vp (wp.v) - wp (vp.v).
+        double ps[R + 2];
 #pragma unroll
         for (int s = 0; s < R; ++s) {
             double acc = 0.0;
@@ -185,18 +195,30 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
             for (int j = 0; j < NC; ++j) acc = fma(a[s][j], v[j], acc);
             ps[s] = wave_sum_dpp(acc);
         }
+        {
+            double d1 = 0.0, d2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                d1 = fma(wp[j], v[j], d1);
+                d2 = fma(vp[j], v[j], d2);
+            }
+            ps[R] = wave_sum_dpp(d1);
+            ps[R + 1] = wave_sum_dpp(d2);
+        }
         if (lane == 0) {
 #pragma unroll
-            for (int s = 0; s < R; ++s) red[wave][s] = ps[s];
+            for (int s = 0; s < R + 2; ++s) red[wave][s] = ps[s];
         }
-        if (g == own1) {
+        if (g == own1) {  // row i+1 of A^(i)
 #pragma unroll
             for (int s = 0; s < R; ++s)
                 if (s == s1) {
+                    const double vs = vrp[s], ws = wrp[s];
 #pragma unroll
                     for (int j = 0; j < NC; ++j)
                         if (c[j] > i && c[j] < n)
-                            __hip_atomic_store(&xr[c[j]], a[s][j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            __hip_atomic_store(&xr[c[j]], upd2(a[s][j], vs, ws, vp[j], wp[j]), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
                 }
         }
         __syncthreads();
@@ -204,9 +226,14 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
             double share = 0.0;
             if (lane < R) {
                 const int r = g + P * lane;
-                double pv = red[0][lane];
+                double pv = red[0][lane], wv = red[0][R], vv = red[0][R + 1];
 #pragma unroll
-                for (int ww = 1; ww < W; ++ww) pv += red[ww][lane];
+                for (int ww = 1; ww < W; ++ww) {
+                    pv += red[ww][lane];
+                    wv += red[ww][R];
+                    vv += red[ww][R + 1];
+                }
+                pv = fma(-wrp[lane], vv, fma(-vrp[lane], wv, pv));
                 pv = (r > i && r < n) ? tau * pv : 0.0;
                 __hip_atomic_store(&xl[LW * g + lane], pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ploc[lane] = pv;
@@ -222,6 +249,13 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
         __syncthreads();
         if (t == 0)
             __hip_atomic_store(&q.flags[g * q.fstride], (unsigned)(i + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the pending update of step i-1, while the flags travel
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const double vs = vrp[s], ws = wrp[s];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) a[s][j] = upd2(a[s][j], vs, ws, vp[j], wp[j]);
+        }
 
         // ---- 2. wait for every workgroup's step-i data
         if (wave == 0 && !dead) {
@@ -262,7 +296,6 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
 #pragma unroll
         for (int j = 0; j < NC; ++j) w[j] = fma(alpha, v[j], pc[j]);
         const double w1 = fma(alpha, 1.0, p1);
-        if (t < R) wr[t] = fma(alpha, vr[cur][t], ploc[t]);
 #pragma unroll
         for (int j = 0; j < NC; ++j) x[j] = (c[j] > i && c[j] < n) ? upd2(rc[j], 1.0, w1, v[j], w[j]) : 0.0;
         if (g == 0) {
@@ -280,17 +313,25 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
             for (int j = 0; j < NC; ++j) vn[j] = 0.0;
             __syncthreads();
         }
+        // step i's update becomes pending; row i+1 is finished (output form)
+        if (t < R) {
+            const int r = g + P * t;
+            vrp[t] = r > i + 1 ? vr[cur][t] : 0.0;
+            wrp[t] = r > i + 1 ? fma(alpha, vr[cur][t], ploc[t]) : 0.0;
+        }
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-            if (g == own1 && s == s1) {
+        for (int j = 0; j < NC; ++j) {
+            vp[j] = v[j];
+            wp[j] = w[j];
+        }
+        if (g == own1) {
 #pragma unroll
-                for (int j = 0; j < NC; ++j)
-                    if (c[j] > i && c[j] < n) a[s][j] = c[j] == i + 1 ? x[j] : (c[j] == i + 2 ? betan : vn[j]);
-            } else {
-                const double vs = vr[cur][s], ws = wr[s];
+            for (int s = 0; s < R; ++s)
+                if (s == s1) {
 #pragma unroll
-                for (int j = 0; j < NC; ++j) a[s][j] = upd2(a[s][j], vs, ws, v[j], w[j]);
-            }
+                    for (int j = 0; j < NC; ++j)
+                        if (c[j] > i && c[j] < n) a[s][j] = c[j] == i + 1 ? x[j] : (c[j] == i + 2 ? betan : vn[j]);
+                }
         }
         const int nxt = cur ^ 1;
 #pragma unroll
