@@ -367,6 +367,52 @@ def study_adjoint(ctx, args):
                       "dEdJ": g.tolist()}), flush=True)
 
 
+def bench_eigensolver(ctx, sizes=(1024, 2048), reps=3):
+    """Symmetric eigensolver of the Gram split: ct_syevd (cooperative tridiagonalization + rocSOLVER dstedc +
+    blocked back-transformation) against rocSOLVER dsyevd on Gram matrices theta theta^T, theta [n x n] with
+    column scales 2^(-i/20) (decaying spectrum, as in a two-site split)."""
+    import ctypes
+    import torch
+    from chemtensor_mojo_amd._lib import call
+    vp = lambda t: ctypes.c_void_p(t.data_ptr())
+    out = {}
+    for n in sizes:
+        g = torch.Generator(device="cuda").manual_seed(n)
+        th = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+        th = th * (2.0 ** (-torch.arange(n, device="cuda", dtype=torch.float64) / 20.0))
+        G = th @ th.T
+        row = {}
+        res = {}
+        for name, method in (("ours", 0), ("rocsolver_dsyevd", 1)):
+            w = torch.zeros(n, dtype=torch.float64, device="cuda")
+            Z = torch.zeros(n, n, dtype=torch.float64, device="cuda")
+            best = 1e9
+            for _ in range(reps):
+                A = G.clone()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                call("ct_syevd", ctx.handle, n, vp(A), vp(w), vp(Z), method)
+                best = min(best, time.perf_counter() - t0)
+            row[name + "_ms"] = round(best * 1e3, 3)
+            res[name] = (w.clone(), Z.clone())
+        w, Z = res["ours"]
+        scale = float(w.abs().max())
+        row["speedup"] = round(row["rocsolver_dsyevd_ms"] / row["ours_ms"], 3)
+        row["max_eig_diff_rel"] = float((w - res["rocsolver_dsyevd"][0]).abs().max()) / scale
+        row["residual_rel"] = float((G @ Z.T - Z.T * w).abs().max()) / scale
+        row["orthogonality"] = float((Z @ Z.T - torch.eye(n, dtype=torch.float64, device="cuda")).abs().max())
+        log(f"eigensolver n={n}: ours {row['ours_ms']:.2f} ms, rocSOLVER dsyevd {row['rocsolver_dsyevd_ms']:.2f} ms, "
+            f"eig diff {row['max_eig_diff_rel']:.1e}, residual {row['residual_rel']:.1e}, orth {row['orthogonality']:.1e}")
+        out[str(n)] = row
+    return out
+
+
+def study_eigensolver(ctx, args):
+    """The Gram-split eigensolver alone at n = 2048 (the two-site split at D = 1024, d = 2), ours against rocSOLVER
+    dsyevd; used for the kernel profile of the eigensolver."""
+    print(json.dumps({"study": "eigensolver", "by_n": bench_eigensolver(ctx, sizes=(2048,))}), flush=True)
+
+
 def cpu_baseline(D, w, budget_s):
     from oracle import oracle as ora
     ora.build()
@@ -398,7 +444,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--study", choices=("stages", "precision", "adjoint"), default=None,
+    ap.add_argument("--study", choices=("stages", "precision", "adjoint", "eigensolver"), default=None,
                     help="run one of the larger studies instead of the default benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -412,11 +458,13 @@ def main():
     D, w = args.D, args.w
     log(f"device: {torch.cuda.get_device_name(0)}")
     if args.study:
-        {"stages": study_stages, "precision": study_precision, "adjoint": study_adjoint}[args.study](ctx, args)
+        {"stages": study_stages, "precision": study_precision, "adjoint": study_adjoint,
+         "eigensolver": study_eigensolver}[args.study](ctx, args)
         ctx.close()
         return
     mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)
     gm, Fg = bench_dominant_gemm(ctx, D, w, args.steps, args.warmup)
+    eig = bench_eigensolver(ctx)
     sweep = None
     if not args.no_sweep:
         s64 = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_FP64, 0)
@@ -468,6 +516,7 @@ def main():
         "roofline_fp64": {"bound": "mfma", "achieved": round(gm["fp64"]["tflops"], 3), "peak": PEAK["fp64"],
                           "unit": "TFLOP/s", "frac": round(gm["fp64"]["tflops"] / PEAK["fp64"], 4)},
         "sweep": sweep,
+        "eigensolver": {"workload": bench_eigensolver.__doc__.replace("\n", " ").replace("    ", " "), "by_n": eig},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -68,7 +68,7 @@ SIGNATURES = {
     "ct_svd_trunc": [_vp, _vp, _i64, _i64, _i64, _dbl, _vp, _vp, _vp, _i64p, _dp],
     "ct_theta_split": [_vp, _int, _vp, _i64, _i64, _i64, _i64, _dbl, _int, _vp, _vp, _i64p, _dp],
     "ct_sytrd": [_vp, _i64, _vp, _vp, _vp, _vp],
-    "ct_syevd": [_vp, _i64, _vp, _vp, _vp],
+    "ct_syevd": [_vp, _i64, _vp, _vp, _vp, _int],
     "ct_dmrg_create": [_vp, ctypes.POINTER(DMRGParamsC), _int, _i64, _i64p, ctypes.POINTER(_vp), _i64p,
                        ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "ct_dmrg_destroy": [_vp],

@@ -351,13 +351,14 @@ int ct_sytrd(ct_handle_t h, int64_t n, double* A, double* d, double* e, double* 
     return 0;
 }
 
-int ct_syevd(ct_handle_t h, int64_t n, double* A, double* w, double* Z) {
+int ct_syevd(ct_handle_t h, int64_t n, double* A, double* w, double* Z, int method) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
     CT_ARG(n >= 1, 2, "n must be >= 1");
+    CT_ARG(method == CT_EIG_AUTO || method == CT_EIG_ROCSOLVER, 6, "unknown method");
     void* ws;
     CT_CHECK(ct::ws_get(h, ct::WS_TMP4, 1024 * 8, &ws));
     int* info = (int*)ws;
-    CT_CHECK(ct::syevd(h, A, n, w, Z, info));
+    CT_CHECK(ct::syevd(h, A, n, w, Z, info, method == CT_EIG_ROCSOLVER));
     int hinfo[2] = {0, 0};
     CT_HIP(hipMemcpyAsync(hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost, h->stream));
     CT_HIP(hipStreamSynchronize(h->stream));

@@ -512,7 +512,8 @@ int bt_prepare(Handle* h, const double* A, const double* tau, int64_t n) {
 // tridiagonalization's arrival wait timed out.  Above n = 2048 (or when the
 // cooperative grid cannot be resident) it runs rocSOLVER dsyevd instead and
 // sets *staged = false: Z then already holds the eigenvectors of A.
-int syevd_stage1(Handle* h, double* A, int64_t n, double* w, double* Z, int* info, bool* staged) {
+int syevd_stage1(Handle* h, double* A, int64_t n, double* w, double* Z, int* info, bool* staged,
+                 bool force_rocsolver) {
     void* wsb;
     CT_CHECK(ws_get(h, WS_EIG2, (size_t)(2 * n + 8) * 8, &wsb));
     double* e = (double*)wsb;
@@ -520,7 +521,7 @@ int syevd_stage1(Handle* h, double* A, int64_t n, double* w, double* Z, int* inf
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, h->stream, (unsigned*)info, 2);
     CT_HIP(hipGetLastError());
     bool ok = false;
-    CT_CHECK(sytrd_coop(h, A, n, w, e, tau, info + 1, &ok));
+    if (!force_rocsolver) CT_CHECK(sytrd_coop(h, A, n, w, e, tau, info + 1, &ok));
     rocblas_status st;
     if (ok) {
         st = rocsolver_dstedc(h->blas, rocblas_evect_tridiagonal, (rocblas_int)n, w, e, Z, (rocblas_int)n,
@@ -575,9 +576,9 @@ int syevd_stage2(Handle* h, int64_t n, double* Z, int64_t rows) {
     return 0;
 }
 
-int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info) {
+int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info, bool force_rocsolver) {
     bool staged = false;
-    CT_CHECK(syevd_stage1(h, A, n, w, Z, info, &staged));
+    CT_CHECK(syevd_stage1(h, A, n, w, Z, info, &staged, force_rocsolver));
     if (staged) CT_CHECK(syevd_stage2(h, n, Z, n));
     return 0;
 }

@@ -284,12 +284,14 @@ int sytrd_coop(Handle* h, double* A, int64_t n, double* d, double* e, double* ta
 // eigenvalues w[n]; eigenvector j in row j of Z[n][n].  info[0] (device):
 // rocSOLVER info; info[1]: tridiagonalization wait timeout.  Both are read by
 // the caller after its next stream synchronisation.
-int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info);
+// force_rocsolver: rocSOLVER dsyevd regardless of n (comparison runs).
+int syevd(Handle* h, double* A, int64_t n, double* w, double* Z, int* info, bool force_rocsolver = false);
 // The same in two stages, so that a caller can read the eigenvalues first and
 // back-transform only the eigenvectors it keeps: stage 1 leaves the
 // eigenvectors of the tridiagonal matrix in Z (final ones when *staged is
 // false); stage 2 turns rows n-rows..n-1 of Z into eigenvectors of A.
-int syevd_stage1(Handle* h, double* A, int64_t n, double* w, double* Z, int* info, bool* staged);
+int syevd_stage1(Handle* h, double* A, int64_t n, double* w, double* Z, int* info, bool* staged,
+                 bool force_rocsolver = false);
 int syevd_stage2(Handle* h, int64_t n, double* Z, int64_t rows);
 
 }  // namespace ct

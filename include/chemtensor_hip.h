@@ -38,8 +38,11 @@ enum {
 /* element type of buffers */
 enum { CT_F64 = 0, CT_F32 = 1 };
 /* two-site split: full SVD (rocSOLVER gesvd), or eigendecomposition of the
- * Gram matrix theta theta^T / theta^T theta (rocSOLVER syevd + 2 GEMMs) */
+ * Gram matrix theta theta^T / theta^T theta (ct_syevd's path + 2 GEMMs) */
 enum { CT_SVD_GESVD = 0, CT_SVD_GRAM = 1 };
+/* ct_syevd methods: the library's path (cooperative tridiagonalization for
+ * n <= 2048) or rocSOLVER dsyevd throughout (for comparison). */
+enum { CT_EIG_AUTO = 0, CT_EIG_ROCSOLVER = 1 };
 
 enum {
     CT_STATUS_OK = 0,
@@ -194,11 +197,12 @@ int ct_theta_split(ct_handle_t h, int svd_mode, const double* theta, int64_t Dl,
  * view of A: d[n], e[n-1], tau[n-1]; row k of A holds d_k at column k, e_k at
  * column k+1 and the reflector v_k at columns k+2..n-1.  Synchronises.
  * ct_syevd: A (destroyed) -> ascending eigenvalues w[n] and eigenvector j in
- * row j of Z[n][n] (LAPACK's column-major eigenvector matrix); ct_sytrd +
- * rocSOLVER dstedc + dormtr for n <= 2048, rocSOLVER dsyevd above.
+ * row j of Z[n][n] (LAPACK's column-major eigenvector matrix).  method
+ * CT_EIG_AUTO: ct_sytrd + rocSOLVER dstedc + a blocked back-transformation
+ * for n <= 2048, rocSOLVER dsyevd above; CT_EIG_ROCSOLVER: rocSOLVER dsyevd.
  * Synchronises. */
 int ct_sytrd(ct_handle_t h, int64_t n, double* A, double* d, double* e, double* tau);
-int ct_syevd(ct_handle_t h, int64_t n, double* A, double* w, double* Z);
+int ct_syevd(ct_handle_t h, int64_t n, double* A, double* w, double* Z, int method);
 
 /* ---- sweep driver ------------------------------------------------------
  * dmrg_two_site (src/algorithms/dmrg.mojo:377-483) with DMRGWorkspace

@@ -105,14 +105,15 @@ def test_sytrd_vs_lapack(ctx, n, kind):
     assert np.abs(lam - np.linalg.eigvalsh(a)).max() <= tol
 
 
-@pytest.mark.parametrize("n,kind", [(1, "sym"), (2, "gram"), (100, "degenerate"), (513, "sym"), (1024, "degenerate"),
-                                    (2048, "gram"), (2100, "sym")])
-def test_syevd_vs_numpy(ctx, n, kind):
+@pytest.mark.parametrize("n,kind,method", [(1, "sym", 0), (2, "gram", 0), (100, "degenerate", 0), (513, "sym", 0),
+                                           (1024, "degenerate", 0), (2048, "gram", 0), (2100, "sym", 0),
+                                           (300, "gram", 1)])
+def test_syevd_vs_numpy(ctx, n, kind, method):
     a = {"sym": random_sym, "gram": gram, "degenerate": degenerate}[kind](n, 5 * n + 3)
     A = torch.tensor(a, dtype=torch.float64, device="cuda").contiguous()
     w = torch.zeros(n, dtype=torch.float64, device="cuda")
     Z = torch.zeros(n, n, dtype=torch.float64, device="cuda")
-    call("ct_syevd", ctx.handle, n, vp(A), vp(w), vp(Z))
+    call("ct_syevd", ctx.handle, n, vp(A), vp(w), vp(Z), method)
     w = w.cpu().numpy()
     Z = Z.cpu().numpy()  # row j = eigenvector j
     scale = np.abs(a).max()
