@@ -79,6 +79,33 @@ __device__ __forceinline__ double block_sum_fixed(double v, double* red) {
     return s;
 }
 
+// Sums of NV values per thread over the NT threads of the workgroup: a DPP
+// butterfly inside each row of 16 lanes, then one LDS pass in which thread v
+// (v < NV, all in wave 0) adds the NT/16 row sums of value v.  The sum of
+// value v is returned in lane v of wave 0 (other lanes: unspecified).
+template <int NT, int NV>
+__device__ __forceinline__ double block_sums_to_lanes(double (&x)[NV], double* part) {
+    constexpr int G = NT / 16;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        x[v] += dpp_mov64<0xB1>(x[v]);   // quad_perm [1,0,3,2]
+        x[v] += dpp_mov64<0x4E>(x[v]);   // quad_perm [2,3,0,1]
+        x[v] += dpp_mov64<0x141>(x[v]);  // row_half_mirror
+        x[v] += dpp_mov64<0x140>(x[v]);  // row_mirror
+    }
+    if ((threadIdx.x & 15) == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) part[v * G + (threadIdx.x >> 4)] = x[v];
+    }
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x < NV) {
+#pragma unroll
+        for (int k = 0; k < G; ++k) s += part[threadIdx.x * G + k];
+    }
+    return s;
+}
+
 // LAPACK dlarfg on the row slice x[c], c >= k+1 (alpha = x[k+1]): v[k+1] = 1,
 // v[c] = x[c] / (alpha - beta) for c >= k+2, zero elsewhere.  Every thread of
 // every workgroup gets the same tau, beta, v.
@@ -118,7 +145,7 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
     constexpr int LW = R + 1 <= 16 ? 16 : 32;  // doubles per exchange line (p[R], share)
     const int n = q.n, P = gridDim.x, g = blockIdx.x, t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    __shared__ double red[W][R + 2];
+    __shared__ double part[(R + 2) * (NT / 16)];
     __shared__ double redw[W];
     __shared__ double bcast;
     __shared__ double ploc[R], vr[2][R], vrp[R], wrp[R];
@@ -193,21 +220,14 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
             double acc = 0.0;
 #pragma unroll
             for (int j = 0; j < NC; ++j) acc = fma(a[s][j], v[j], acc);
-            ps[s] = wave_sum_dpp(acc);
+            ps[s] = acc;
         }
-        {
-            double d1 = 0.0, d2 = 0.0;
+        ps[R] = 0.0;
+        ps[R + 1] = 0.0;
 #pragma unroll
-            for (int j = 0; j < NC; ++j) {
-                d1 = fma(wp[j], v[j], d1);
-                d2 = fma(vp[j], v[j], d2);
-            }
-            ps[R] = wave_sum_dpp(d1);
-            ps[R + 1] = wave_sum_dpp(d2);
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int s = 0; s < R + 2; ++s) red[wave][s] = ps[s];
+        for (int j = 0; j < NC; ++j) {
+            ps[R] = fma(wp[j], v[j], ps[R]);
+            ps[R + 1] = fma(vp[j], v[j], ps[R + 1]);
         }
         if (g == own1) {  // row i+1 of A^(i)
 #pragma unroll
@@ -221,19 +241,13 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
                                                __HIP_MEMORY_SCOPE_AGENT);
                 }
         }
-        __syncthreads();
+        const double sums = block_sums_to_lanes<NT, R + 2>(ps, part);  // lane s of wave 0: row s, R: wp.v, R+1: vp.v
         if (wave == 0) {
+            const double wv = readlane64(sums, R), vv = readlane64(sums, R + 1);
             double share = 0.0;
             if (lane < R) {
                 const int r = g + P * lane;
-                double pv = red[0][lane], wv = red[0][R], vv = red[0][R + 1];
-#pragma unroll
-                for (int ww = 1; ww < W; ++ww) {
-                    pv += red[ww][lane];
-                    wv += red[ww][R];
-                    vv += red[ww][R + 1];
-                }
-                pv = fma(-wrp[lane], vv, fma(-vrp[lane], wv, pv));
+                double pv = fma(-wrp[lane], vv, fma(-vrp[lane], wv, sums));
                 pv = (r > i && r < n) ? tau * pv : 0.0;
                 __hip_atomic_store(&xl[LW * g + lane], pv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ploc[lane] = pv;
