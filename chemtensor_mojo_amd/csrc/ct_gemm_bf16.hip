@@ -105,6 +105,46 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ X, int
     }
 }
 
+// Transposing split (rows contiguous, s_row == 1) with 16-byte stores: a
+// 32 (k) x 64 (rows) tile goes through LDS; each thread then splits 8
+// consecutive k of one row and writes their hi and lo halves as one 16-byte
+// store each (4 threads fill a row's 128-byte line).  Needs ld % 8 == 0 and a
+// 16-byte aligned output base.
+template <typename T>
+__global__ __launch_bounds__(256) void split_t_kernel(const T* __restrict__ X, int rows, int K, int64_t s_k,
+                                                      int64_t bsX, uint16_t* __restrict__ out, int64_t ld, int64_t bs,
+                                                      const int* active) {
+    if (active && *active == 0) return;
+    __shared__ float tile[32][65];
+    const int b = blockIdx.z;
+    const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 32;  // neighbouring blocks read neighbouring rows
+    const int t = threadIdx.x;
+    X += (int64_t)b * bsX;
+    out += (int64_t)b * bs;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int kk = (t >> 6) + 4 * i, k = k0 + kk, r = r0 + (t & 63);
+        tile[kk][t & 63] = (k < K && r < rows) ? (float)X[(int64_t)r + (int64_t)k * s_k] : 0.f;
+    }
+    __syncthreads();
+    const int row = t >> 2, grp = t & 3;
+    const int r = r0 + row;
+    if (r < rows) {
+        uint32_t hp[4], lp[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint16_t h0, l0, h1, l1;
+            split_one(tile[grp * 8 + 2 * q][row], h0, l0);
+            split_one(tile[grp * 8 + 2 * q + 1][row], h1, l1);
+            hp[q] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            lp[q] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+        }
+        uint16_t* o = out + (int64_t)r * ld + 2 * k0 + grp * 8;
+        *(v4u*)o = v4u{hp[0], hp[1], hp[2], hp[3]};
+        *(v4u*)(o + 32) = v4u{lp[0], lp[1], lp[2], lp[3]};
+    }
+}
+
 struct NtArgs {
     const uint16_t *A, *B;  // split rows
     void* C;
@@ -382,6 +422,17 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
 int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
                int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active) {
     if (rows <= 0 || K <= 0 || batch <= 0) return 0;
+    if (s_row == 1 && ld_out % 8 == 0 && bs_out % 8 == 0 && ((uintptr_t)out % 16) == 0) {
+        dim3 grid((unsigned)((rows + 63) / 64), (unsigned)(split_k32(K) / 32), (unsigned)batch), block(256);
+        if (tX == CT_F64)
+            hipLaunchKernelGGL(split_t_kernel<double>, grid, block, 0, h->stream, (const double*)X, rows, K, s_k, bsX,
+                               out, ld_out, bs_out, active);
+        else
+            hipLaunchKernelGGL(split_t_kernel<float>, grid, block, 0, h->stream, (const float*)X, rows, K, s_k, bsX,
+                               out, ld_out, bs_out, active);
+        CT_HIP(hipGetLastError());
+        return 0;
+    }
     dim3 grid((unsigned)(split_k32(K) / 32), (unsigned)((rows + 31) / 32), (unsigned)batch), block(256);
     if (tX == CT_F64)
         hipLaunchKernelGGL(split_kernel<double>, grid, block, 0, h->stream, (const double*)X, rows, K, s_row, s_k,

@@ -69,6 +69,40 @@ def test_tensordot_mixed_vs_oracle(ctx, ora, prec, tol):
         assert rel(C.to_numpy(), exp) < tol, (sA, sB)
 
 
+def _bf16_rne(x32):
+    """float32 array -> bf16 bit patterns, round to nearest even (inf/nan truncated)."""
+    u = x32.view(np.uint32).astype(np.uint64)
+    special = (u & 0x7F800000) == 0x7F800000
+    r = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return np.where(special, u >> 16, r).astype(np.uint16)
+
+
+@pytest.mark.parametrize("rows,K,extra,dt", [(4096, 1024, 0, np.float64), (100, 70, 8, np.float64),
+                                             (257, 33, 4, np.float32), (64, 32, 0, np.float32)])
+def test_split_bf16_transposing(ctx, rows, K, extra, dt):
+    """ct_split_bf16 of a matrix stored with rows contiguous (s_row = 1), as the
+    H_eff path splits theta and the environments: bit-exact against a numpy
+    restatement of hi = bf16(x), lo = bf16(x - hi) with zero padding to 32.
+    extra = 4 takes the path for row strides that are not a multiple of 8."""
+    from chemtensor_mojo_amd._lib import call
+    import ctypes
+    rng = np.random.default_rng(rows + K)
+    X = rng.standard_normal((K, rows)).astype(dt)  # element (r, k) at X[k][r]
+    K32 = (K + 31) // 32 * 32
+    ld = 2 * K32 + extra
+    xt = torch.tensor(X, device="cuda")
+    out = torch.full((rows * ld,), 0x7FC0, dtype=torch.int16, device="cuda")
+    call("ct_split_bf16", ctx.handle, 0 if dt == np.float64 else 1, ctypes.c_void_p(xt.data_ptr()), rows, K, 1, rows,
+         ctypes.c_void_p(out.data_ptr()), ld)
+    got = out.cpu().numpy().view(np.uint16).reshape(rows, ld)[:, :2 * K32].reshape(rows, K32 // 32, 2, 32)
+    x32 = np.zeros((rows, K32), np.float32)
+    x32[:, :K] = X.T.astype(np.float32)
+    hi = _bf16_rne(x32)
+    lo = _bf16_rne(x32 - (hi.astype(np.uint32) << 16).view(np.float32))
+    assert np.array_equal(got[:, :, 0, :], hi.reshape(rows, K32 // 32, 32))
+    assert np.array_equal(got[:, :, 1, :], lo.reshape(rows, K32 // 32, 32))
+
+
 @pytest.mark.parametrize("M,N,K,dt,extra", [
     (2048, 2048, 4096, torch.float64, 0),   # 256x256 direct-to-LDS tiling, split-K = 4, FP64 output
     (4000, 4100, 200, torch.float32, 64),   # 256x256 tiling, ragged M/N, K padded 200 -> 224, ld > 2*K32
