@@ -149,15 +149,17 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
     __shared__ double redw[W];
     __shared__ double bcast;
     __shared__ double ploc[R], vr[2][R], vrp[R], wrp[R];
-    __shared__ int dead;
+    __shared__ double redp[W];
+    bool dead = false;  // this thread's arrival wait timed out once: stop waiting
 
     // Column c[j] of this thread; where its p value sits in a step's
     // exchange region; which of this workgroup's rows it is (-1: none).
-    int c[NC], poff[NC], lrow[NC];
+    int c[NC], poff[NC], lin[NC], lrow[NC];
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
         c[j] = t + NT * j;
-        poff[j] = LW * (c[j] % P) + c[j] / P;
+        lin[j] = c[j] % P;
+        poff[j] = LW * lin[j] + c[j] / P;
         const int dl = c[j] - g;
         lrow[j] = (c[j] < n && dl >= 0 && dl % P == 0 && dl / P < R) ? dl / P : -1;
     }
@@ -169,7 +171,6 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
         for (int j = 0; j < NC; ++j) a[s][j] = (r < n && c[j] < n) ? q.A[(int64_t)r * q.lda + c[j]] : 0.0;
     }
     if (t < R) { vr[0][t] = 0.0; vr[1][t] = 0.0; vrp[t] = 0.0; wrp[t] = 0.0; }
-    if (t == 0) dead = 0;
 
     // reflector 0 from row 0 of the input
     double v[NC], x[NC], w[NC];
@@ -271,41 +272,46 @@ __global__ __launch_bounds__(NT, 1) void sytrd_coop_kernel(TrArgs q) {
             for (int j = 0; j < NC; ++j) a[s][j] = upd2(a[s][j], vs, ws, vp[j], wp[j]);
         }
 
-        // ---- 2. wait for every workgroup's step-i data
-        if (wave == 0 && !dead) {
-            const unsigned target = (unsigned)(i + 1);
+        // ---- 2. every thread waits only for the writers of the data it reads
+        // and loads each piece as soon as its flag is seen: p of its columns
+        // (line c mod P), the share of line t (t < P), row i+1 (its owner).
+        // The lines are write-once, so a load issued after the flag returns
+        // the published value; the workgroup-scope acquire after each wait
+        // keeps the compiler from hoisting the load above the poll.
+        double pc[NC], rc[NC];
+        double shr = 0.0;
+        const unsigned target = (unsigned)(i + 1);
+        auto wait_flag = [&](int m) {
             int polls = 0;
-            while (true) {
-                bool ok = true;
-                for (int m = lane; m < P; m += 64)
-                    ok = ok && __hip_atomic_load(&q.flags[m * q.fstride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
-                                   target;
-                if (__all(ok)) break;
+            while (!dead && __hip_atomic_load(&q.flags[m * q.fstride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                                target) {
                 if (++polls > q.spin_limit) {
-                    if (lane == 0) {
-                        dead = 1;
-                        atomicExch(q.err, 1);
-                    }
-                    break;
+                    dead = true;
+                    atomicExch(q.err, 1);
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-        }
-        // loads below stay after the poll (write-once lines: no cache invalidation needed)
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        __syncthreads();
-
-        // ---- 3. w, row i+1 after step i, reflector i+1, local update
-        double pdotv = 0.0;  // same summation order in every wave of every workgroup
-        for (int m = lane; m < P; m += 64) pdotv += xl[LW * m + R];
-        double pc[NC], rc[NC];
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        };
+        wait_flag(own1);
+        const double p1 = xl[LW * own1 + s1];
+#pragma unroll
+        for (int j = 0; j < NC; ++j) rc[j] = (c[j] > i && c[j] < n) ? xr[c[j]] : 0.0;
 #pragma unroll
         for (int j = 0; j < NC; ++j) {
-            pc[j] = (c[j] > i && c[j] < n) ? xl[poff[j]] : 0.0;
-            rc[j] = (c[j] > i && c[j] < n) ? xr[c[j]] : 0.0;
+            pc[j] = 0.0;
+            if (c[j] > i && c[j] < n) {
+                wait_flag(lin[j]);
+                pc[j] = xl[poff[j]];
+            }
         }
-        const double p1 = xl[LW * own1 + s1];
-        pdotv = wave_sum_dpp(pdotv);
+        if (t < P) {
+            wait_flag(t);
+            shr = xl[LW * t + R];
+        }
+
+        // ---- 3. w, row i+1 after step i, reflector i+1, local update
+        const double pdotv = block_sum_fixed<NT>(shr, redp);  // same order in every workgroup
         const double alpha = -0.5 * tau * pdotv;
 #pragma unroll
         for (int j = 0; j < NC; ++j) w[j] = fma(alpha, v[j], pc[j]);
