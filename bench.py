@@ -57,7 +57,9 @@ def make_problem(D, w, seed=42):
     L = 0.5 * (L + L.transpose(0, 2, 1))
     R = rng.standard_normal((w, D, D)) / np.sqrt(D)
     R = 0.5 * (R + R.transpose(0, 2, 1))
-    W = ham.j1j2_tensors(6, 1.0, 0.5)[2]  # bulk tensor [8, 2, 2, 8]
+    # bulk tensors: J1-J2 [8, 2, 2, 8] or XXZ (J=1, Delta=1) [5, 2, 2, 5]
+    W = ham.j1j2_tensors(6, 1.0, 0.5)[2] if w == 8 else ham.heisenberg_xxz_tensors(6, 1.0, 1.0, 0.0)[2]
+    assert W.shape[0] == w
     th = rng.standard_normal((D, 2, 2, D))
     th /= np.linalg.norm(th)
     return L, R, W, th
@@ -112,6 +114,34 @@ def bench_matvec(ctx, D, w, steps, warmup):
     del tL, tR, tth
     torch.cuda.empty_cache()
     return res, F
+
+
+def bench_matvec_grid(ctx, steps, warmup, Ds=(256, 512, 1024), ws=(5, 8)):
+    """Prepared-operator matvec rate (TFLOP/s, mean over `steps` calls timed with HIP events) over D and w: the
+    microbenchmark grid (XXZ bulk MPO for w=5, J1-J2 bulk MPO for w=8)."""
+    import torch
+    import chemtensor_mojo_amd as ct
+    grid = {}
+    for w in ws:
+        for D in Ds:
+            L, R, W, th = make_problem(D, w)
+            dev = lambda a: ct.create_dense_tensor_from_data(ctx, a, a.shape)
+            tL, tR, tW, tth = dev(L), dev(R), ct.MPOSite(dev(W)), dev(th)
+            F = matvec_flops(D, 2, D, w, w, w)
+            row = {"gflop": round(F / 1e9, 3)}
+            for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3)):
+                op = ct.HeffOperator(tL, tR, tW, tW, ctx, prec=prec)
+                buf = torch.empty(D * 4 * D, dtype=torch.float64, device="cuda")
+                t, _ = time_events(lambda: op.apply(tth, buf), steps, warmup)
+                op.close()
+                row[name + "_ms"] = round(t * 1e3, 4)
+                row[name + "_tflops"] = round(F / t / 1e12, 2)
+            log(f"matvec grid w={w} D={D}: fp64 {row['fp64_tflops']} fp32 {row['fp32_tflops']} bf16x3 "
+                f"{row['bf16x3_tflops']} TFLOP/s")
+            grid[f"w{w}_D{D}"] = row
+            del tL, tR, tth
+            torch.cuda.empty_cache()
+    return grid
 
 
 def bench_dominant_gemm(ctx, D, w, steps, warmup):
@@ -465,6 +495,7 @@ def main():
     mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)
     gm, Fg = bench_dominant_gemm(ctx, D, w, args.steps, args.warmup)
     eig = bench_eigensolver(ctx)
+    mgrid = bench_matvec_grid(ctx, max(5, args.steps // 2), args.warmup)
     sweep = None
     if not args.no_sweep:
         s64 = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_FP64, 0)
@@ -517,6 +548,7 @@ def main():
                           "unit": "TFLOP/s", "frac": round(gm["fp64"]["tflops"] / PEAK["fp64"], 4)},
         "sweep": sweep,
         "eigensolver": {"workload": bench_eigensolver.__doc__.replace("\n", " ").replace("    ", " "), "by_n": eig},
+        "matvec_grid": {"workload": bench_matvec_grid.__doc__.replace("\n", " ").replace("    ", " "), "by_w_D": mgrid},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

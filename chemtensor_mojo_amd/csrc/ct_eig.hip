@@ -408,30 +408,34 @@ __global__ void build_vt_kernel(const double* __restrict__ A, const double* __re
 }
 
 // T (upper triangular) of one block from S = V^T V: T(i,i) = tau_i,
-// T(0:i, i) = -tau_i T(0:i, 0:i) S(0:i, i).  One workgroup per block.
+// T(0:i, i) = -tau_i T(0:i, 0:i) S(0:i, i).  One workgroup per block; row r of
+// column i is summed by the 4 lanes of a quad (q = r + l, r + l + 4, ...).
 __global__ __launch_bounds__(256) void larft_kernel(const double* __restrict__ S, const double* __restrict__ taup,
                                                     double* __restrict__ T) {
     constexpr int B = BT_B;
     __shared__ double s[B][B + 1], tt[B][B + 1];
-    const int m = blockIdx.x, r = threadIdx.x;
+    const int m = blockIdx.x;
+    const int r = threadIdx.x >> 2, l = threadIdx.x & 3;
     const double* Sm = S + (int64_t)m * B * B;
-    for (int t = r; t < B * B; t += blockDim.x) {
+    for (int t = threadIdx.x; t < B * B; t += blockDim.x) {
         s[t / B][t % B] = Sm[t];
         tt[t / B][t % B] = 0.0;
     }
     __syncthreads();
     for (int i = 0; i < B; ++i) {
         const double ti = taup[m * B + i];
-        if (r < i) {  // reads columns < i of tt, writes column i
-            double acc = 0.0;
-            for (int q = r; q < i; ++q) acc = fma(tt[r][q], s[q][i], acc);
-            tt[r][i] = -ti * acc;
+        double acc = 0.0;  // reads columns < i of tt, writes column i
+        for (int q = r + l; q < i; q += 4) acc = fma(tt[r][q], s[q][i], acc);
+        acc += dpp_mov64<0xB1>(acc);
+        acc += dpp_mov64<0x4E>(acc);
+        if (l == 0) {
+            if (r < i) tt[r][i] = -ti * acc;
+            if (r == i) tt[i][i] = ti;
         }
-        if (r == i) tt[i][i] = ti;
         __syncthreads();
     }
     double* Tm = T + (int64_t)m * B * B;
-    for (int t = r; t < B * B; t += blockDim.x) Tm[t] = tt[t / B][t % B];
+    for (int t = threadIdx.x; t < B * B; t += blockDim.x) Tm[t] = tt[t / B][t % B];
 }
 
 template <int NT, int R, int NC>
