@@ -13,6 +13,8 @@
 // current one run.  Operands may be m/n-contiguous or k-contiguous (the
 // transposes the reference performs with separate permute kernels,
 // dense_tensor.mojo:1249-1257, happen here inside the tile load).
+#include <algorithm>
+
 #include "ct_internal.h"
 #include "ct_device.h"
 
@@ -53,6 +55,9 @@ struct KArgs {
     int64_t sAm, sAk, sBk, sBn, ldc, bsA, bsB, bsC;
     double alpha, beta;
     const int* active;
+    int splits;  // split-K factor (blockIdx.z = batch * splits + split)
+    int kchunk;  // K range per split, a multiple of BK
+    void* part;  // [batch*splits][M][N] partial sums (TC) when splits > 1
 };
 
 constexpr int BM = 128, BN = 128, BK = 16;
@@ -149,7 +154,10 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     int tm, tn;
     gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
-    const int z = blockIdx.z;
+    const int zz = blockIdx.z;
+    const int z = zz / p.splits;
+    const int kbeg = (zz - z * p.splits) * p.kchunk;
+    const int kend = min(p.K, kbeg + p.kchunk);
     const TA* A = (const TA*)p.A + (int64_t)z * p.bsA;
     const TB* B = (const TB*)p.B + (int64_t)z * p.bsB;
     TO* C = (TO*)p.C + (int64_t)z * p.bsC;
@@ -167,18 +175,18 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     TC ra[8], rb[8];
     const int64_t sAx = AKC ? p.sAm : 0, sAk = AKC ? 0 : p.sAk;
     const int64_t sBx = BKC ? p.sBn : 0, sBk = BKC ? 0 : p.sBk;
-    load_tile<TA, TC, AKC, FULL>(A, sAx, sAk, m0, p.M, 0, p.K, p.vecA, ra);
-    load_tile<TB, TC, BKC, FULL>(B, sBx, sBk, n0, p.N, 0, p.K, p.vecB, rb);
+    load_tile<TA, TC, AKC, FULL>(A, sAx, sAk, m0, p.M, kbeg, kend, p.vecA, ra);
+    load_tile<TB, TC, BKC, FULL>(B, sBx, sBk, n0, p.N, kbeg, kend, p.vecB, rb);
     store_tile<TC, AKC, BM + PAD>(sA[0], ra);
     store_tile<TC, BKC, BN + PAD>(sB[0], rb);
     __syncthreads();
 
     int cur = 0;
-    for (int k0 = 0; k0 < p.K; k0 += BK) {
-        const bool more = k0 + BK < p.K;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        const bool more = k0 + BK < kend;
         if (more) {
-            load_tile<TA, TC, AKC, FULL>(A, sAx, sAk, m0, p.M, k0 + BK, p.K, p.vecA, ra);
-            load_tile<TB, TC, BKC, FULL>(B, sBx, sBk, n0, p.N, k0 + BK, p.K, p.vecB, rb);
+            load_tile<TA, TC, AKC, FULL>(A, sAx, sAk, m0, p.M, k0 + BK, kend, p.vecA, ra);
+            load_tile<TB, TC, BKC, FULL>(B, sBx, sBk, n0, p.N, k0 + BK, kend, p.vecB, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
@@ -200,6 +208,22 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
         cur ^= 1;
     }
 
+    if (p.splits > 1) {  // raw partial sums of this K range; the reduction applies alpha / beta
+        TC* P = (TC*)p.part + (int64_t)zz * p.M * p.N;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + wn * 64 + j * 16 + li;
+                if (!FULL && n >= p.N) continue;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm * 64 + i * 16 + Mfma<TC>::row(lk, r);
+                    if (FULL || m < p.M) P[(int64_t)m * p.N + n] = acc[i][j][r];
+                }
+            }
+        return;
+    }
     const double alpha = p.alpha, beta = p.beta;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -218,6 +242,25 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
                 }
             }
         }
+}
+
+// C[b][m][n] = alpha * sum_s part[b][s][m][n] (summed in FP64) + beta * C
+template <typename TP, typename TO>
+__global__ __launch_bounds__(256) void gemm_splitk_reduce(const TP* __restrict__ part, TO* __restrict__ C, int M, int N,
+                                                          int64_t ldc, int64_t bsC, int splits, int batch, double alpha,
+                                                          double beta, const int* active) {
+    if (active && *active == 0) return;
+    const int64_t MN = (int64_t)M * N, total = MN * batch;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t / MN, e = t - b * MN;
+        const int64_t m = e / N, n = e - m * N;
+        double s = 0.0;
+        for (int k = 0; k < splits; ++k) s += (double)part[(b * splits + k) * MN + e];
+        TO* c = C + b * bsC + m * ldc + n;
+        double v = alpha * s;
+        if (beta != 0.0) v += beta * (double)(*c);
+        *c = (TO)v;
+    }
 }
 
 template <typename T>
@@ -240,7 +283,20 @@ int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDes
     const bool bkc = g.b_kc != 0;
     p.vecA = akc ? aligned16<TA>(A, g.sAm, g.bsA) : aligned16<TA>(A, g.sAk, g.bsA);
     p.vecB = bkc ? aligned16<TB>(B, g.sBn, g.bsB) : aligned16<TB>(B, g.sBk, g.bsB);
-    dim3 grid(p.tilesM * p.tilesN, 1, g.batch), block(256);
+    // Split K when the tiles alone would leave most CUs idle (small bond
+    // dimensions): at least 8 K-tiles per split, aiming at >= 256 workgroups.
+    const int64_t tiles = (int64_t)p.tilesM * p.tilesN * g.batch;
+    int splits = 1;
+    if (tiles < 128 && g.K >= 16 * BK) splits = (int)std::min<int64_t>((256 + tiles - 1) / tiles, g.K / (8 * BK));
+    p.kchunk = g.K;
+    if (splits > 1) {
+        p.kchunk = (int)(((g.K + splits - 1) / splits + BK - 1) / BK * BK);
+        splits = (g.K + p.kchunk - 1) / p.kchunk;
+    }
+    p.splits = splits;
+    p.part = nullptr;
+    if (splits > 1) CT_CHECK(ws_get(h, WS_SPLITK, (size_t)g.batch * splits * g.M * g.N * sizeof(TC), &p.part));
+    dim3 grid(p.tilesM * p.tilesN, 1, g.batch * splits), block(256);
     const bool full = p.vecA && p.vecB && (g.M % BM == 0) && (g.N % BN == 0) && (g.K % BK == 0);
 #define CT_LAUNCH(AK, BKc, F) \
     hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, AK, BKc, F>), grid, block, 0, h->stream, p)
@@ -257,6 +313,13 @@ int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDes
     }
 #undef CT_LAUNCH
     CT_HIP(hipGetLastError());
+    if (splits > 1) {
+        const int64_t total = (int64_t)g.batch * g.M * g.N;
+        const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+        hipLaunchKernelGGL((gemm_splitk_reduce<TC, TO>), dim3(blocks), dim3(256), 0, h->stream, (const TC*)p.part,
+                           (TO*)C, g.M, g.N, g.ldc, g.bsC, splits, g.batch, g.alpha, g.beta, g.active);
+        CT_HIP(hipGetLastError());
+    }
     return 0;
 }
 

@@ -41,6 +41,10 @@ DOT_CASES = [
     ((300, 257), (300, 190), 1, True, True),
     ((190, 33), (77, 33), 1, False, False),
     ((64, 256), (128, 64), 1, True, False),
+    # few output tiles, long K: split-K partial sums + reduction
+    ((64, 4096), (4096, 96), 1, False, True),
+    ((3, 5000), (7, 5000), 1, False, False),
+    ((2, 3, 1000), (3, 1000, 130), 2, False, True),
 ]
 
 
