@@ -349,7 +349,10 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
     int tm, tn;
     gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
     const int m0 = tm * SM, n0 = tn * SM;
-    const int z = blockIdx.z;
+    const int zz = blockIdx.z;  // batch * splits + split
+    const int z = zz / p.splits;
+    const int kbeg = (zz - z * p.splits) * p.kchunk;
+    const int kend = min(p.K32, kbeg + p.kchunk);
     const uint16_t* A = p.A + (int64_t)z * p.bsA;
     const uint16_t* B = p.B + (int64_t)z * p.bsB;
 
@@ -364,15 +367,15 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
         for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
     v4u ra[4], rb[4];
-    nt_load(A, p.lda, m0, p.M, 0, ra);
-    nt_load(B, p.ldb, n0, p.N, 0, rb);
+    nt_load(A, p.lda, m0, p.M, kbeg, ra);
+    nt_load(B, p.ldb, n0, p.N, kbeg, rb);
     nt_store(sAh[0], sAl[0], ra);
     nt_store(sBh[0], sBl[0], rb);
     __syncthreads();
 
     int buf = 0;
-    for (int k0 = 0; k0 < p.K32; k0 += BK) {
-        const bool more = k0 + BK < p.K32;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        const bool more = k0 + BK < kend;
         if (more) {
             nt_load(A, p.lda, m0, p.M, k0 + BK, ra);
             nt_load(B, p.ldb, n0, p.N, k0 + BK, rb);
@@ -406,6 +409,21 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
         buf ^= 1;
     }
 
+    if (p.splits > 1) {  // FP32 partial sums of this K range, reduced by splitk_reduce_kernel
+        float* P = p.part + (int64_t)zz * p.M * p.N;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + wn * 64 + j * 16 + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm * 64 + i * 16 + lk * 4 + r;
+                    if (m < p.M && n < p.N) P[(int64_t)m * p.N + n] = acc[i][j][r];
+                }
+            }
+        return;
+    }
     TO* C = (TO*)p.C + (int64_t)z * p.bsC;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -511,11 +529,34 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
     }
     p.tilesM = (M + SM - 1) / SM;
     p.tilesN = (N + SM - 1) / SM;
-    dim3 grid(p.tilesM * p.tilesN, 1, batch), block(256);
-    if (tC == CT_F64)
-        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, grid, block, 0, h->stream, p);
-    else
+    // split K when the 128x128 tiles leave most CUs idle (at least 8 K blocks per split)
+    const int64_t t128 = (int64_t)p.tilesM * p.tilesN * batch;
+    int s128 = 1;
+    if (t128 < 128 && K32 >= 16 * BK) s128 = (int)std::min<int64_t>((256 + t128 - 1) / t128, K32 / (8 * BK));
+    if (s128 > 1) {
+        p.kchunk = ((K32 + s128 - 1) / s128 + BK - 1) / BK * BK;
+        s128 = (K32 + p.kchunk - 1) / p.kchunk;
+    }
+    p.splits = s128;
+    dim3 grid(p.tilesM * p.tilesN, 1, batch * s128), block(256);
+    if (s128 > 1) {
+        void* w;
+        CT_CHECK(ws_get(h, WS_SPLITK, (size_t)batch * s128 * M * N * 4, &w));
+        p.part = (float*)w;
         hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<float>, grid, block, 0, h->stream, p);
+        const int64_t total = (int64_t)M * N * batch;
+        const int g = (int)std::min<int64_t>((total + 255) / 256, 8192);
+        if (tC == CT_F64)
+            hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(g), dim3(256), 0, h->stream, (const float*)p.part,
+                               (double*)C, M, N, s128, batch, ldc, bsC, alpha, beta, active);
+        else
+            hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(g), dim3(256), 0, h->stream, (const float*)p.part,
+                               (float*)C, M, N, s128, batch, ldc, bsC, alpha, beta, active);
+    } else if (tC == CT_F64) {
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, grid, block, 0, h->stream, p);
+    } else {
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<float>, grid, block, 0, h->stream, p);
+    }
     CT_HIP(hipGetLastError());
     return 0;
 }

@@ -109,6 +109,8 @@ def test_split_bf16_transposing(ctx, rows, K, extra, dt):
 
 @pytest.mark.parametrize("M,N,K,dt,extra", [
     (2048, 2048, 4096, torch.float64, 0),   # 256x256 direct-to-LDS tiling, split-K = 4, FP64 output
+    (64, 96, 4096, torch.float64, 0),       # 128x128 tiling, one tile: split-K with the FP32 reduction
+    (130, 70, 1500, torch.float32, 8),      # 128x128 tiling, ragged, split-K over a padded K
     (4000, 4100, 200, torch.float32, 64),   # 256x256 tiling, ragged M/N, K padded 200 -> 224, ld > 2*K32
     (300, 190, 77, torch.float64, 0),       # 128x128 tiling, ragged
 ])
