@@ -68,7 +68,7 @@ struct Handle {
     bool own_stream = false;
     rocblas_handle blas = nullptr;
     // named workspaces (H_eff intermediates, reductions, SVD, Lanczos basis ...)
-    Buffer ws[32];
+    Buffer ws[40];
     double* d_scalars = nullptr;  // small FP64 scratch for reductions (pinned-size)
     double* h_pinned = nullptr;   // pinned host mirror for scalar read-back
     int num_cu = 256;
@@ -107,7 +107,9 @@ enum WsSlot {
     WS_EIG3 = 29,    // back-transformation: reflector rows Vt
     WS_EIG4 = 30,    // back-transformation: T^T Vt per block
     WS_EIG5 = 31,    // back-transformation: V^T V and T per block
-    WS_COUNT = 32
+    WS_MIXA = 32,    // large-w mixing: packed input
+    WS_MIXB = 33,    // large-w mixing: GEMM output
+    WS_COUNT = 34
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);

@@ -2,6 +2,8 @@
 // (the K = w*d contractions of apply_two_site_heff steps 2-3 and of the
 // environment updates), generic permutes, conversions, fills and FP64
 // wavefront reductions.
+#include <algorithm>
+
 #include "ct_internal.h"
 #include "ct_device.h"
 
@@ -202,11 +204,6 @@ static inline int grid_for(int64_t n) {
 namespace {
 
 int mix_args(const MixDesc& m, MixArgs* a) {
-    const int K = m.CIN * m.EIN, NO = m.COUT * m.EOUT;
-    if (K > MIX_KMAX || NO > MIX_NMAX) {
-        set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
-        return CT_STATUS_NOT_SUPPORTED;
-    }
     a->CIN = m.CIN; a->EIN = m.EIN; a->COUT = m.COUT; a->EOUT = m.EOUT;
     a->P = m.P; a->Q = m.Q;
     a->sXc = m.sXc; a->sXp = m.sXp; a->sXe = m.sXe;
@@ -241,6 +238,79 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
     return 0;
 }
 
+// ------------------------------------------------ large MPO bond dimension
+// When w*d (environments) or w*d^2 (H_eff) exceeds 64 the per-pixel mixing
+// is a real matrix product: per pixel row p, Y_p[n][q] = sum_k coef[k][n] X_p[k][q]
+// with K, NO up to thousands.  X is packed to [P][K][Q], one batched MFMA
+// GEMM (FP64, or FP32 for FP32 / 3xBF16 contractions) forms [P][NO][Q], and an
+// unpack kernel writes Y's layout (plain, or 3xBF16 split rows).
+template <typename TX, typename TT>
+__global__ void mix_pack_kernel(const TX* __restrict__ X, TT* __restrict__ Xt, MixArgs a) {
+    if (a.active && *a.active == 0) return;
+    const int K = a.CIN * a.EIN;
+    const int64_t total = a.P * (int64_t)K * a.Q;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = t % a.Q, pk = t / a.Q;
+        const int k = (int)(pk % K);
+        const int64_t p = pk / K;
+        const int ci = k / a.EIN, ei = k - ci * a.EIN;
+        Xt[t] = (TT)X[ci * a.sXc + p * a.sXp + ei * a.sXe + q];
+    }
+}
+
+template <typename TT, int OUT>
+__global__ void mix_unpack_kernel(const TT* __restrict__ Yt, void* __restrict__ Yv, MixArgs a) {
+    if (a.active && *a.active == 0) return;
+    const int NO = a.COUT * a.EOUT;
+    const int64_t total = a.P * (int64_t)NO * a.Q;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = t % a.Q, pn = t / a.Q;
+        const int n = (int)(pn % NO);
+        const int64_t p = pn / NO;
+        const int co = n / a.EOUT, eo = n - co * a.EOUT;
+        const double v = (double)Yt[t];
+        if (OUT == 0) {
+            ((double*)Yv)[co * a.sYc + p * a.sYp + eo * a.sYe + q] = v;
+        } else if (OUT == 1) {
+            ((float*)Yv)[co * a.sYc + p * a.sYp + eo * a.sYe + q] = (float)v;
+        } else {
+            const int64_t k = co * a.sYc + q;
+            const int64_t o = eo * a.sYe + p * a.sYp + 2 * (k & ~(int64_t)31) + (k & 31);
+            const float f = (float)v;
+            const uint16_t hb = f2bf(f);
+            const float hf = __uint_as_float(((uint32_t)hb) << 16);
+            ((uint16_t*)Yv)[o] = hb;
+            ((uint16_t*)Yv)[o + 32] = f2bf(f - hf);
+        }
+    }
+}
+
+template <typename TX, typename TT, int OUT>
+int mix_large(Handle* h, const TX* X, void* Y, const double* d_coef, const MixArgs& a) {
+    const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
+    void *wx, *wy;
+    CT_CHECK(ws_get(h, WS_MIXA, (size_t)(a.P * K * a.Q) * sizeof(TT), &wx));
+    CT_CHECK(ws_get(h, WS_MIXB, (size_t)(a.P * NO * a.Q) * sizeof(TT), &wy));
+    const int64_t nin = a.P * (int64_t)K * a.Q, nout = a.P * (int64_t)NO * a.Q;
+    hipLaunchKernelGGL((mix_pack_kernel<TX, TT>), dim3((unsigned)std::min<int64_t>((nin + 255) / 256, 16384)),
+                       dim3(256), 0, h->stream, X, (TT*)wx, a);
+    CT_HIP(hipGetLastError());
+    GemmDesc g;  // Yt[p][n][q] = sum_k coef[k][n] Xt[p][k][q]
+    g.M = NO; g.N = (int)a.Q; g.K = K; g.batch = (int)a.P;
+    g.a_kc = 0; g.sAm = 1; g.sAk = NO; g.bsA = 0;
+    g.b_kc = 0; g.sBk = a.Q; g.sBn = 1; g.bsB = (int64_t)K * a.Q;
+    g.ldc = a.Q; g.bsC = (int64_t)NO * a.Q;
+    g.active = a.active;
+    const int tt = sizeof(TT) == 8 ? CT_F64 : CT_F32;
+    CT_CHECK(gemm(h, tt == CT_F64 ? CT_PREC_FP64 : CT_PREC_FP32, CT_F64, d_coef, tt, wx, tt, wy, g));
+    hipLaunchKernelGGL((mix_unpack_kernel<TT, OUT>), dim3((unsigned)std::min<int64_t>((nout + 255) / 256, 16384)),
+                       dim3(256), 0, h->stream, (const TT*)wy, Y, a);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
+bool mix_is_large(const MixArgs& a) { return a.CIN * a.EIN > MIX_KMAX || a.COUT * a.EOUT > MIX_NMAX; }
+
 }  // namespace
 
 size_t mix_pad_bytes(int K, int NO, int tAcc) {
@@ -249,10 +319,7 @@ size_t mix_pad_bytes(int K, int NO, int tAcc) {
 }
 
 int mix_pad_coef(Handle* h, const double* d_coef, int K, int NO, int tAcc, void* dst) {
-    if (K > MIX_KMAX || NO > MIX_NMAX) {
-        set_last_error("MPO mixing: w*d exceeds 64 (not supported by this kernel)");
-        return CT_STATUS_NOT_SUPPORTED;
-    }
+    if (K > MIX_KMAX || NO > MIX_NMAX) return 0;  // the large-w path reads the FP64 coefficients directly
     const int KM = mix_km(K, NO);
     if (tAcc == CT_F64)
         hipLaunchKernelGGL(pad_coef_kernel<double>, dim3((KM * KM + 255) / 256), dim3(256), 0, h->stream, d_coef,
@@ -269,6 +336,14 @@ int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef,
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
     CT_CHECK(mix_args(m, &a));
+    if (mix_is_large(a)) {
+        if (tX == CT_F64 && tY == CT_F64) return mix_large<double, double, 0>(h, (const double*)X, Y, d_coef, a);
+        if (tX == CT_F64 && tY == CT_F32) return mix_large<double, double, 1>(h, (const double*)X, Y, d_coef, a);
+        if (tX == CT_F32 && tY == CT_F32) return mix_large<float, float, 1>(h, (const float*)X, Y, d_coef, a);
+        if (tX == CT_F32 && tY == CT_F64) return mix_large<float, float, 0>(h, (const float*)X, Y, d_coef, a);
+        set_last_error("mix: unsupported types");
+        return CT_STATUS_NOT_SUPPORTED;
+    }
     if (tX == CT_F64 && tY == CT_F64) {
         CT_CHECK((launch_mix<double, double, 0>(h, (const double*)X, Y, d_coef, a, padded)));
     } else if (tX == CT_F64 && tY == CT_F32) {
@@ -291,6 +366,7 @@ int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, cons
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
     CT_CHECK(mix_args(m, &a));
+    if (mix_is_large(a)) return mix_large<float, float, 2>(h, X, (void*)Y, d_coef, a);
     CT_CHECK((launch_mix<float, float, 2>(h, X, (void*)Y, d_coef, a, padded)));
     CT_HIP(hipGetLastError());
     return 0;

@@ -258,7 +258,9 @@ def _rand_problem(rng, Dl, Dr, d, wL, wM, wR, Dlb=None, Drb=None):
 
 
 HEFF_SHAPES = [(1, 2, 2, 1, 5, 1), (2, 4, 2, 5, 5, 5), (7, 9, 2, 5, 5, 5), (16, 16, 2, 8, 8, 8),
-               (33, 65, 2, 5, 5, 5), (64, 64, 2, 5, 5, 5), (130, 96, 2, 8, 8, 8), (40, 40, 3, 4, 4, 4)]
+               (33, 65, 2, 5, 5, 5), (64, 64, 2, 5, 5, 5), (130, 96, 2, 8, 8, 8), (40, 40, 3, 4, 4, 4),
+               # w*d^2 > 64: mixing as a batched GEMM (quantum-chemistry MPO sizes)
+               (20, 24, 2, 40, 40, 40), (17, 9, 2, 100, 70, 30)]
 
 
 @pytest.mark.parametrize("shape", HEFF_SHAPES)
@@ -283,7 +285,8 @@ def test_heff_bra_dims_differ(ctx, ora):
 
 
 @pytest.mark.parametrize("prec,tol", [(ct.PREC_FP32, 2e-6), (ct.PREC_BF16X3, 6e-5)])
-@pytest.mark.parametrize("shape", [(64, 64, 2, 5, 5, 5), (130, 96, 2, 8, 8, 8), (256, 256, 2, 8, 8, 8)])
+@pytest.mark.parametrize("shape", [(64, 64, 2, 5, 5, 5), (130, 96, 2, 8, 8, 8), (256, 256, 2, 8, 8, 8),
+                                   (64, 48, 2, 40, 40, 40)])
 def test_heff_mixed_vs_oracle(ctx, ora, prec, tol, shape):
     Dl, Dr, d, wL, wM, wR = shape
     rng = np.random.default_rng(7 + Dl)
@@ -295,7 +298,8 @@ def test_heff_mixed_vs_oracle(ctx, ora, prec, tol, shape):
         assert rel(out.to_numpy(), exp) < tol, dt
 
 
-ENV_SHAPES = [(1, 2, 2, 1, 5), (2, 2, 4, 5, 5), (7, 2, 13, 5, 5), (64, 2, 70, 8, 8), (33, 3, 20, 4, 4)]
+ENV_SHAPES = [(1, 2, 2, 1, 5), (2, 2, 4, 5, 5), (7, 2, 13, 5, 5), (64, 2, 70, 8, 8), (33, 3, 20, 4, 4),
+              (20, 2, 24, 40, 40), (9, 2, 11, 100, 70)]
 
 
 @pytest.mark.parametrize("shape", ENV_SHAPES)
