@@ -437,6 +437,49 @@ def bench_eigensolver(ctx, sizes=(1024, 2048), reps=3):
     return out
 
 
+def study_qc(ctx, args):
+    """Quantum-chemistry-shaped DMRG at reduced size: n_orb = 6 spatial orbitals (12 spin-orbital sites, Jordan-Wigner),
+    h_pq symmetric N(0,1), (pq|rs) N(0,0.1) with 8-fold symmetry (seed 42), half filling through a particle-number
+    penalty 1.0 (N - 6)^2 (2.4x the smallest penalty that makes the 6-electron sector lowest: a larger one inflates
+    |H| and with it the mixed-precision matvec error), exact MPO by SVD of the dense operator, chi_max = 64 (exact on
+    12 sites) and chi_max = 32 (truncated), 6 sweeps, 20 Krylov vectors, exact tridiagonal solves; FP64 against FP32
+    and 3xBF16 with FP64 refinement, all against full CI of the 6-electron sector."""
+    import chemtensor_mojo_amd as ct
+    from chemtensor_mojo_amd import hamiltonians as ham
+    n_orb = 6
+    t0 = time.perf_counter()
+    mpo_t, info = ham.qc_tensors(n_orb, seed=42, penalty=1.0)
+    log(f"QC MPO built in {time.perf_counter() - t0:.1f} s; bond dims {[W.shape[3] for W in mpo_t[:-1]]}; "
+        f"E_fci = {info['E_fci']:.12f}")
+    L = 2 * n_orb
+    mpo = ct.mpo_from_tensors(ctx, mpo_t)
+    out = {}
+    for D in (64, 32):
+        for form, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
+                                   ("bf16x3_refined", ct.PREC_BF16X3, 2)):
+            name = f"D{D}_{form}"
+            mps = ct.create_random_mps_c_compatible(ctx, L, 2, D, 42)
+            p = ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=0.0, max_krylov_iter=20, krylov_tol=1e-12,
+                              energy_tol=0.0, jacobi_max_rot=-1, prec=prec, refine=refine)
+            run = ct.DMRGRun(ctx, mpo, mps, p)
+            rows = _sweep_deltas(run, 6, name)
+            run.close()
+            E = rows[-1]["E"]
+            out[name] = {"E": E, "dE_rel_vs_fci": abs(E - info["E_fci"]) / abs(info["E_fci"]),
+                         "trunc_weight_sum": sum(r["trunc_weight_sum"] for r in rows),
+                         "total_s": sum(r["s"] for r in rows), "sweeps": rows}
+        E64 = out[f"D{D}_fp64"]["E"]
+        for form in ("fp64", "fp32_refined", "bf16x3_refined"):
+            o = out[f"D{D}_{form}"]
+            o["dE_rel_vs_fp64"] = abs(o["E"] - E64) / abs(E64)
+            log(f"D={D} {form}: E={o['E']:.14f} dE/E vs FCI {o['dE_rel_vs_fci']:.2e} vs fp64 "
+                f"{o['dE_rel_vs_fp64']:.2e} trunc weight {o['trunc_weight_sum']:.2e} time={o['total_s']:.2f} s")
+    print(json.dumps({"study": "qc", "workload": study_qc.__doc__.replace("\n", " "), "E_fci": info["E_fci"],
+                      "mpo_bond_dims": [int(W.shape[3]) for W in mpo_t[:-1]],
+                      "summary": {k: {kk: vv for kk, vv in v.items() if kk != "sweeps"} for k, v in out.items()},
+                      "sweeps": {k: v["sweeps"] for k, v in out.items()}}), flush=True)
+
+
 def study_eigensolver(ctx, args):
     """The Gram-split eigensolver alone at n = 2048 (the two-site split at D = 1024, d = 2), ours against rocSOLVER
     dsyevd; used for the kernel profile of the eigensolver."""
@@ -474,7 +517,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--study", choices=("stages", "precision", "adjoint", "eigensolver"), default=None,
+    ap.add_argument("--study", choices=("stages", "precision", "adjoint", "eigensolver", "qc"), default=None,
                     help="run one of the larger studies instead of the default benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -489,7 +532,7 @@ def main():
     log(f"device: {torch.cuda.get_device_name(0)}")
     if args.study:
         {"stages": study_stages, "precision": study_precision, "adjoint": study_adjoint,
-         "eigensolver": study_eigensolver}[args.study](ctx, args)
+         "eigensolver": study_eigensolver, "qc": study_qc}[args.study](ctx, args)
         ctx.close()
         return
     mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)

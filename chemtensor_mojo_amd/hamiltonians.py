@@ -149,3 +149,112 @@ def mpo_to_dense(sites: Sequence[np.ndarray]) -> np.ndarray:
         s = M.shape
         M = M.reshape(s[0] * s[1], s[2] * s[3], s[4])
     return M[:, :, 0]
+
+
+# ------------------------------------------------ quantum chemistry (no reference counterpart)
+def qc_integrals(n_orb: int, seed: int = 42, v_scale: float = 0.1) -> tuple[np.ndarray, np.ndarray]:
+    """Random ab initio-shaped integrals over n_orb spatial orbitals: h_pq real
+    symmetric N(0,1); v_pqrs = (pq|rs) N(0, v_scale) with the 8-fold
+    permutational symmetry of real orbitals imposed by averaging."""
+    rng = np.random.default_rng(seed)
+    h = rng.standard_normal((n_orb, n_orb))
+    h = 0.5 * (h + h.T)
+    v = rng.standard_normal((n_orb,) * 4) * v_scale
+    perms = [(0, 1, 2, 3), (1, 0, 2, 3), (0, 1, 3, 2), (1, 0, 3, 2),
+             (2, 3, 0, 1), (3, 2, 0, 1), (2, 3, 1, 0), (3, 2, 1, 0)]
+    v = sum(v.transpose(p) for p in perms) / 8.0
+    return h, v
+
+
+def _jw_annihilators(num_sites: int):
+    """Sparse a_j = Z x .. x Z x sigma^- x I x .. x I (site 0 the most
+    significant bit of the basis index; Z|1> = -|1>, sigma^- = |0><1|)."""
+    import scipy.sparse as sp
+    Zs = sp.csr_matrix(Z)
+    Is = sp.identity(2, format="csr")
+    Ms = sp.csr_matrix(np.array([[0.0, 1.0], [0.0, 0.0]]))
+    ops = []
+    for j in range(num_sites):
+        m = sp.identity(1, format="csr")
+        for i in range(num_sites):
+            m = sp.kron(m, Zs if i < j else (Ms if i == j else Is), format="csr")
+        ops.append(m)
+    return ops
+
+
+def qc_hamiltonian_sparse(h: np.ndarray, v: np.ndarray, penalty: float = 0.0, n_elec: int | None = None):
+    """Second-quantised H = sum_pq h_pq E_pq + 1/2 sum_pqrs (pq|rs) (E_pq E_rs - delta_qr E_ps)
+    (E_pq = sum_sigma a+_{p sigma} a_{q sigma}) on 2*n_orb spin-orbital sites
+    (site 2p = p up, 2p+1 = p down) under the Jordan-Wigner mapping, plus
+    penalty * (N - n_elec)^2 when penalty > 0 (n_elec defaults to half filling).
+    Returns a scipy sparse matrix <out|H|in> of size 4^n_orb (test / small-system use)."""
+    import scipy.sparse as sp
+    n = h.shape[0]
+    M = 2 * n
+    a = _jw_annihilators(M)
+    ad = [x.T.tocsr() for x in a]
+    E = [[sum(ad[2 * p + s] @ a[2 * q + s] for s in (0, 1)) for q in range(n)] for p in range(n)]
+    H = sp.csr_matrix((4 ** n, 4 ** n))
+    for p in range(n):
+        for q in range(n):
+            H = H + h[p, q] * E[p][q]
+    for p in range(n):
+        for q in range(n):
+            F = sp.csr_matrix((4 ** n, 4 ** n))
+            for r in range(n):
+                for s in range(n):
+                    if v[p, q, r, s] != 0.0:
+                        F = F + v[p, q, r, s] * E[r][s]
+            H = H + 0.5 * (E[p][q] @ F)
+    for p in range(n):
+        for s in range(n):
+            H = H - 0.5 * sum(v[p, q, q, s] for q in range(n)) * E[p][s]
+    if penalty > 0.0:
+        ne = n if n_elec is None else n_elec
+        N = sum(ad[j] @ a[j] for j in range(M))
+        Dn = N - ne * sp.identity(4 ** n, format="csr")
+        H = H + penalty * (Dn @ Dn)
+    return H.tocsr()
+
+
+def mpo_from_dense(H: np.ndarray, num_sites: int, tol: float = 1e-12) -> list[np.ndarray]:
+    """Exact MPO of a dense 2^L x 2^L operator <out|H|in> by successive SVDs
+    (tensor-train decomposition of the operator with the site pair index
+    d_in*2 + d_out), dropping singular values below tol * the largest at each
+    cut.  Inverse of mpo_to_dense; small systems only."""
+    L = num_sites
+    T = np.asarray(H, dtype=np.float64).reshape([2] * (2 * L))
+    order = []
+    for k in range(L):
+        order += [L + k, k]  # (in_k, out_k)
+    C = T.transpose(order).reshape(4, -1)
+    sites, r_prev = [], 1
+    for k in range(L - 1):
+        C = C.reshape(r_prev * 4, -1)
+        U, S, Vt = np.linalg.svd(C, full_matrices=False)
+        r = max(1, int((S > tol * S[0]).sum())) if S[0] > 0 else 1
+        sites.append(U[:, :r].reshape(r_prev, 2, 2, r))
+        C = S[:r, None] * Vt[:r]
+        r_prev = r
+    sites.append(C.reshape(r_prev, 2, 2, 1))
+    return sites
+
+
+def qc_tensors(n_orb: int, seed: int = 42, penalty: float = 10.0, n_elec: int | None = None,
+               v_scale: float = 0.1) -> tuple[list[np.ndarray], dict]:
+    """MPO tensors of the penalised quantum-chemistry Hamiltonian on 2*n_orb
+    sites (exact, by SVD of the dense operator; n_orb <= 6) and its exact
+    ground state: {"E_fci", "E_penalised", "n_elec"} with E_fci the lowest
+    eigenvalue of H in the n_elec sector (half filling by default)."""
+    import scipy.sparse as sp
+    h, v = qc_integrals(n_orb, seed, v_scale)
+    ne = n_orb if n_elec is None else n_elec
+    Hs = qc_hamiltonian_sparse(h, v, penalty, ne)
+    Hd = Hs.toarray()
+    M = 2 * n_orb
+    occ = np.array([bin(i).count("1") for i in range(2 ** M)])
+    sector = np.where(occ == ne)[0]
+    H0 = qc_hamiltonian_sparse(h, v, 0.0).toarray()
+    e_fci = float(np.linalg.eigvalsh(H0[np.ix_(sector, sector)])[0])
+    e_pen = float(np.linalg.eigvalsh(Hd)[0])
+    return mpo_from_dense(Hd, M), {"E_fci": e_fci, "E_penalised": e_pen, "n_elec": ne, "h": h, "v": v}

@@ -549,3 +549,23 @@ def test_dmrg_heisenberg_afm_vs_oracle_and_ed(ctx, ora, svd_mode):
     e0 = np.linalg.eigvalsh(H)[0]
     assert abs(E - ref["E"]) <= 1e-10 * abs(ref["E"])
     assert abs(E - e0) < 1e-8 * abs(e0)
+
+
+# ----------------------------------------------------- quantum chemistry
+@pytest.mark.parametrize("prec,refine,tol", [(ct.PREC_FP64, 0, 1e-10), (ct.PREC_FP32, 2, 1e-9),
+                                             (ct.PREC_BF16X3, 2, 1e-9)])
+def test_dmrg_quantum_chemistry_vs_fci(ctx, ora, prec, refine, tol):
+    """4 spatial orbitals (8 spin-orbital sites), random integrals, half filling
+    through a particle-number penalty, exact MPO (bond dimensions up to 46, so
+    the MPO mixing runs as a batched GEMM); chi_max = 16 is exact on 8 sites, so
+    every form must reach the full-CI energy of the half-filling sector."""
+    mpo_t, info = ham.qc_tensors(4, seed=21, penalty=1.0)
+    mps_t = ct.random_mps_c_compatible_tensors(8, 2, 16, 5)
+    p = ct.DMRGParams(num_sweeps=6, chi_max=16, eps_trunc=0.0, max_krylov_iter=20, krylov_tol=1e-12,
+                      energy_tol=0.0, jacobi_max_rot=-1, prec=prec, refine=refine)
+    E, out, st = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p,
+                                  return_stats=True)
+    assert abs(E - info["E_fci"]) <= tol * abs(info["E_fci"]), (E, info["E_fci"])
+    if prec == ct.PREC_FP64:
+        ref = ora.dmrg_two_site(mps_t, mpo_t, 6, 16, 0.0, 20, 1e-12, 0.0, jacobi_max_rot=20000)
+        assert abs(E - ref["E"]) <= 1e-10 * abs(ref["E"])
