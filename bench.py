@@ -438,26 +438,28 @@ def bench_eigensolver(ctx, sizes=(1024, 2048), reps=3):
 
 
 def study_qc(ctx, args):
-    """Quantum-chemistry-shaped DMRG at reduced size: n_orb = 6 spatial orbitals (12 spin-orbital sites, Jordan-Wigner),
-    h_pq symmetric N(0,1), (pq|rs) N(0,0.1) with 8-fold symmetry (seed 42), half filling through a particle-number
-    penalty 1.0 (N - 6)^2 (2.4x the smallest penalty that makes the 6-electron sector lowest: a larger one inflates
-    |H| and with it the mixed-precision matvec error), exact MPO by SVD of the dense operator, chi_max = 64 (exact on
-    12 sites) and chi_max = 32 (truncated), 6 sweeps, 20 Krylov vectors, exact tridiagonal solves; FP64 against FP32
-    and 3xBF16 with FP64 refinement, all against full CI of the 6-electron sector."""
+    """Quantum-chemistry-shaped DMRG at reduced sizes: n_orb = 6 and 8 spatial orbitals (12 and 16 spin-orbital sites,
+    Jordan-Wigner), h_pq symmetric N(0,1), (pq|rs) N(0,0.1) with 8-fold symmetry (seed 42), half filling through a
+    particle-number penalty 1.0 (N - n_orb)^2 (2.4x the smallest penalty that makes the half-filled sector lowest at
+    n_orb = 6: a larger one inflates |H| and with it the mixed-precision matvec error), exact MPO from the
+    Jordan-Wigner operator strings; chi_max = 64 (exact on 12 sites) and 32 (truncated) at n_orb = 6, chi_max = 256
+    (exact on 16 sites) at n_orb = 8; 6 sweeps, 20 Krylov vectors, exact tridiagonal solves; FP64 against FP32 and
+    3xBF16 with FP64 refinement, all against full CI of the half-filled sector."""
     import chemtensor_mojo_amd as ct
     from chemtensor_mojo_amd import hamiltonians as ham
-    n_orb = 6
-    t0 = time.perf_counter()
-    mpo_t, info = ham.qc_tensors(n_orb, seed=42, penalty=1.0)
-    log(f"QC MPO built in {time.perf_counter() - t0:.1f} s; bond dims {[W.shape[3] for W in mpo_t[:-1]]}; "
-        f"E_fci = {info['E_fci']:.12f}")
-    L = 2 * n_orb
-    mpo = ct.mpo_from_tensors(ctx, mpo_t)
-    out = {}
-    for D in (64, 32):
+    out, problems = {}, {}
+    for n_orb, D in ((6, 64), (6, 32), (8, 256)):
+        if n_orb not in problems:
+            t0 = time.perf_counter()
+            mpo_t, info = ham.qc_tensors(n_orb, seed=42, penalty=1.0, exact=True)
+            log(f"QC MPO n_orb={n_orb} and full CI in {time.perf_counter() - t0:.1f} s; bond dims "
+                f"{[W.shape[3] for W in mpo_t[:-1]]}; E_fci = {info['E_fci']:.12f}")
+            problems[n_orb] = (ct.mpo_from_tensors(ctx, mpo_t), info, [int(W.shape[3]) for W in mpo_t[:-1]])
+        mpo, info, _ = problems[n_orb]
+        L = 2 * n_orb
         for form, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
                                    ("bf16x3_refined", ct.PREC_BF16X3, 2)):
-            name = f"D{D}_{form}"
+            name = f"n{n_orb}_D{D}_{form}"
             mps = ct.create_random_mps_c_compatible(ctx, L, 2, D, 42)
             p = ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=0.0, max_krylov_iter=20, krylov_tol=1e-12,
                               energy_tol=0.0, jacobi_max_rot=-1, prec=prec, refine=refine)
@@ -465,19 +467,95 @@ def study_qc(ctx, args):
             rows = _sweep_deltas(run, 6, name)
             run.close()
             E = rows[-1]["E"]
-            out[name] = {"E": E, "dE_rel_vs_fci": abs(E - info["E_fci"]) / abs(info["E_fci"]),
+            out[name] = {"E": E, "E_fci": info["E_fci"], "dE_rel_vs_fci": abs(E - info["E_fci"]) / abs(info["E_fci"]),
                          "trunc_weight_sum": sum(r["trunc_weight_sum"] for r in rows),
                          "total_s": sum(r["s"] for r in rows), "sweeps": rows}
-        E64 = out[f"D{D}_fp64"]["E"]
+        E64 = out[f"n{n_orb}_D{D}_fp64"]["E"]
         for form in ("fp64", "fp32_refined", "bf16x3_refined"):
-            o = out[f"D{D}_{form}"]
+            o = out[f"n{n_orb}_D{D}_{form}"]
             o["dE_rel_vs_fp64"] = abs(o["E"] - E64) / abs(E64)
-            log(f"D={D} {form}: E={o['E']:.14f} dE/E vs FCI {o['dE_rel_vs_fci']:.2e} vs fp64 "
+            log(f"n_orb={n_orb} D={D} {form}: E={o['E']:.14f} dE/E vs FCI {o['dE_rel_vs_fci']:.2e} vs fp64 "
                 f"{o['dE_rel_vs_fp64']:.2e} trunc weight {o['trunc_weight_sum']:.2e} time={o['total_s']:.2f} s")
-    print(json.dumps({"study": "qc", "workload": study_qc.__doc__.replace("\n", " "), "E_fci": info["E_fci"],
-                      "mpo_bond_dims": [int(W.shape[3]) for W in mpo_t[:-1]],
+    print(json.dumps({"study": "qc", "workload": study_qc.__doc__.replace("\n", " "),
+                      "mpo_bond_dims": {str(k): v[2] for k, v in problems.items()},
                       "summary": {k: {kk: vv for kk, vv in v.items() if kk != "sweeps"} for k, v in out.items()},
                       "sweeps": {k: v["sweeps"] for k, v in out.items()}}), flush=True)
+
+
+def _mpo_expectation(mps_sites, mpo_sites):
+    """<psi|O|psi> / <psi|psi> of an MPO over host MPS tensors [Dl, d, Dr]."""
+    E = np.ones((1, 1, 1))
+    N = np.ones((1, 1))
+    for A, W in zip(mps_sites, mpo_sites):
+        T = np.tensordot(E, A, axes=([1], [0]))          # [a, l', i, r]
+        T = np.tensordot(T, W, axes=([0, 2], [0, 1]))    # [l', r, o, b]
+        E = np.tensordot(T, A, axes=([0, 2], [0, 1]))    # [r, b, r']
+        E = E.transpose(1, 0, 2)
+        N = np.tensordot(np.tensordot(N, A, axes=([0], [0])), A, axes=([0, 1], [0, 1]))
+    return float(E[0, 0, 0] / N[0, 0])
+
+
+def study_qc16(ctx, args):
+    """Quantum-chemistry-shaped DMRG at full size: n_orb = 16 spatial orbitals (32 spin-orbital sites, Jordan-Wigner),
+    h_pq symmetric N(0,1), (pq|rs) N(0,0.1) with 8-fold symmetry (seed 42), half filling (16 electrons) through the
+    particle-number penalty 1.0 (N - 16)^2, exact MPO from the operator strings (central bond dimension 562),
+    PCG32 random start D=16, chi_max = 512, eps_trunc = 1e-12, 6 sweeps, 20 Krylov vectors, exact tridiagonal solves;
+    FP64 against FP32 and 3xBF16 with FP64 refinement.  Then one more sweep in each form, and in FP64 with the same
+    refinement, from the same FP64 state (isolates the arithmetic from the different trajectories of truncated runs),
+    and <(N - 16)^2> of each result."""
+    import chemtensor_mojo_amd as ct
+    from chemtensor_mojo_amd import hamiltonians as ham
+    n_orb, D, sweeps, mu = 16, 512, int(args.qc_sweeps), float(args.qc_penalty)
+    t0 = time.perf_counter()
+    h, v = ham.qc_integrals(n_orb, 42)
+    mpo_t = ham.qc_mpo_tensors(h, v, mu, n_orb)
+    pen_t = ham.qc_mpo_tensors(np.zeros_like(h), np.zeros_like(v), 1.0, n_orb)
+    log(f"QC MPO built in {time.perf_counter() - t0:.1f} s; bond dims {[W.shape[3] for W in mpo_t[:-1]]}")
+    L = 2 * n_orb
+    mpo = ct.mpo_from_tensors(ctx, mpo_t)
+    forms = (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2), ("bf16x3_refined", ct.PREC_BF16X3, 2))
+
+    def params(prec, refine, krylov=20):
+        return ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=1e-12, max_krylov_iter=krylov, krylov_tol=1e-12,
+                             energy_tol=0.0, jacobi_max_rot=-1, prec=prec, refine=refine)
+
+    out, warm, fp64_state = {}, {}, None
+    for form, prec, refine in forms:
+        run = ct.DMRGRun(ctx, mpo, ct.create_random_mps_c_compatible(ctx, L, 2, 16, 42), params(prec, refine))
+        rows = _sweep_deltas(run, sweeps, form)
+        state = run.mps()
+        run.close()
+        host = state.to_numpy()
+        o = {"E": rows[-1]["E"], "trunc_weight_sum": sum(r["trunc_weight_sum"] for r in rows),
+             "total_s": sum(r["s"] for r in rows), "max_bond": rows[-1]["max_bond"],
+             "penalty_expectation": _mpo_expectation(host, pen_t), "sweeps": rows}
+        out[form] = o
+        log(f"{form}: E={o['E']:.13f} <(N-16)^2>={o['penalty_expectation']:.2e} "
+            f"trunc weight {o['trunc_weight_sum']:.2e} time={o['total_s']:.1f} s")
+        if form == "fp64":
+            fp64_state = state
+        else:
+            del state
+    E64 = out["fp64"]["E"]
+    for form in out:
+        out[form]["dE_rel_vs_fp64"] = abs(out[form]["E"] - E64) / abs(E64)
+    warm_forms = (forms[0], ("fp64_refined", ct.PREC_FP64, 2)) + forms[1:]
+    for form, prec, refine in warm_forms:
+        run = ct.DMRGRun(ctx, mpo, fp64_state, params(prec, refine))
+        rows = _sweep_deltas(run, 1, f"warm {form}")
+        run.close()
+        warm[form] = {"E": rows[-1]["E"], "s": rows[-1]["s"], "matvecs": rows[-1]["matvecs"],
+                      "trunc_weight_sum": rows[-1]["trunc_weight_sum"]}
+    W64, W64r = warm["fp64"]["E"], warm["fp64_refined"]["E"]
+    for form in warm:
+        warm[form]["dE_rel_vs_fp64"] = abs(warm[form]["E"] - W64) / abs(W64)
+        warm[form]["dE_rel_vs_fp64_refined"] = abs(warm[form]["E"] - W64r) / abs(W64r)
+        log(f"warm {form}: E={warm[form]['E']:.13f} dE/E vs fp64 {warm[form]['dE_rel_vs_fp64']:.2e} "
+            f"vs fp64 refined {warm[form]['dE_rel_vs_fp64_refined']:.2e} {warm[form]['s']:.1f} s")
+    print(json.dumps({"study": "qc16", "workload": study_qc16.__doc__.replace("\n", " "),
+                      "mpo_bond_dims": [int(W.shape[3]) for W in mpo_t[:-1]],
+                      "summary": {k: {kk: vv for kk, vv in v.items() if kk != "sweeps"} for k, v in out.items()},
+                      "warm_sweep": warm, "sweeps": {k: v["sweeps"] for k, v in out.items()}}), flush=True)
 
 
 def study_eigensolver(ctx, args):
@@ -517,7 +595,9 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--study", choices=("stages", "precision", "adjoint", "eigensolver", "qc"), default=None,
+    ap.add_argument("--qc-sweeps", type=int, default=6)
+    ap.add_argument("--qc-penalty", type=float, default=1.0)
+    ap.add_argument("--study", choices=("stages", "precision", "adjoint", "eigensolver", "qc", "qc16"), default=None,
                     help="run one of the larger studies instead of the default benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -532,7 +612,7 @@ def main():
     log(f"device: {torch.cuda.get_device_name(0)}")
     if args.study:
         {"stages": study_stages, "precision": study_precision, "adjoint": study_adjoint,
-         "eigensolver": study_eigensolver, "qc": study_qc}[args.study](ctx, args)
+         "eigensolver": study_eigensolver, "qc": study_qc, "qc16": study_qc16}[args.study](ctx, args)
         ctx.close()
         return
     mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)

@@ -195,20 +195,21 @@ def qc_hamiltonian_sparse(h: np.ndarray, v: np.ndarray, penalty: float = 0.0, n_
     ad = [x.T.tocsr() for x in a]
     E = [[sum(ad[2 * p + s] @ a[2 * q + s] for s in (0, 1)) for q in range(n)] for p in range(n)]
     H = sp.csr_matrix((4 ** n, 4 ** n))
+    flat = [E[p][q].tocoo() for p in range(n) for q in range(n)]
+
+    def lincomb(c):  # sum_pq c[p, q] E_pq, assembled once from coordinates
+        c = np.asarray(c).ravel()
+        nz = [k for k in range(c.size) if c[k] != 0.0]
+        if not nz:
+            return sp.csr_matrix((4 ** n, 4 ** n))
+        return sp.coo_matrix((np.concatenate([c[k] * flat[k].data for k in nz]),
+                              (np.concatenate([flat[k].row for k in nz]), np.concatenate([flat[k].col for k in nz]))),
+                             shape=(4 ** n, 4 ** n)).tocsr()
+
+    H = lincomb(h - 0.5 * np.einsum("pqqs->ps", v))
     for p in range(n):
         for q in range(n):
-            H = H + h[p, q] * E[p][q]
-    for p in range(n):
-        for q in range(n):
-            F = sp.csr_matrix((4 ** n, 4 ** n))
-            for r in range(n):
-                for s in range(n):
-                    if v[p, q, r, s] != 0.0:
-                        F = F + v[p, q, r, s] * E[r][s]
-            H = H + 0.5 * (E[p][q] @ F)
-    for p in range(n):
-        for s in range(n):
-            H = H - 0.5 * sum(v[p, q, q, s] for q in range(n)) * E[p][s]
+            H = H + 0.5 * (E[p][q] @ lincomb(v[p, q]))
     if penalty > 0.0:
         ne = n if n_elec is None else n_elec
         N = sum(ad[j] @ a[j] for j in range(M))
@@ -240,21 +241,188 @@ def mpo_from_dense(H: np.ndarray, num_sites: int, tol: float = 1e-12) -> list[np
     return sites
 
 
+# Local operator basis of the operator strings: I, Z, c+ = |1><0|, c = |0><1|
+# (letters 0..3; |1> = occupied).  Every 2x2 matrix is a unique combination.
+_ANN = np.array([[0.0, 1.0], [0.0, 0.0]])
+_LETTERS = np.stack([I2, Z, _ANN.T, _ANN])
+
+
+def _qc_ladder_terms(h: np.ndarray, v: np.ndarray, penalty: float, n_elec: int):
+    """The Hamiltonian of qc_hamiltonian_sparse as a list of products of at
+    most four ladder operators: (modes [T, 4], creator [T, 4], coef [T]);
+    mode -1 pads shorter products.  Two-body part in normal order:
+    1/2 sum (pq|rs) a+_{p s} a+_{r t} a_{s t} a_{q s}; penalty
+    mu (N - n)^2 = mu [sum_i (1 - 2n) n_i + sum_{i != j} n_i n_j + n^2]."""
+    n = h.shape[0]
+    M = 2 * n
+    modes, cre, coefs = [], [], []
+
+    def add(mo, cr, c):
+        modes.append(mo)
+        cre.append(cr)
+        coefs.append(c)
+
+    p, q, s = (x.ravel() for x in np.meshgrid(np.arange(n), np.arange(n), np.arange(2), indexing="ij"))
+    mo = np.full((p.size, 4), -1)
+    mo[:, 0], mo[:, 1] = 2 * p + s, 2 * q + s
+    cr = np.zeros((p.size, 4), bool)
+    cr[:, 0] = True
+    add(mo, cr, h[p, q])
+    p, q, r, t, s1, s2 = (x.ravel() for x in np.meshgrid(*([np.arange(n)] * 4 + [np.arange(2)] * 2),
+                                                        indexing="ij"))
+    mo = np.stack([2 * p + s1, 2 * r + s2, 2 * t + s2, 2 * q + s1], 1)
+    c = 0.5 * v[p, q, r, t]
+    keep = (mo[:, 0] != mo[:, 1]) & (mo[:, 2] != mo[:, 3]) & (c != 0.0)
+    cr = np.zeros((int(keep.sum()), 4), bool)
+    cr[:, :2] = True
+    add(mo[keep], cr, c[keep])
+    if penalty > 0.0:
+        i, j = (x.ravel() for x in np.meshgrid(np.arange(M), np.arange(M), indexing="ij"))
+        off = i != j
+        mo = np.stack([i[off], i[off], j[off], j[off]], 1)
+        cr = np.zeros(mo.shape, bool)
+        cr[:, 0] = cr[:, 2] = True
+        add(mo, cr, np.full(mo.shape[0], penalty))
+        mo = np.full((M, 4), -1)
+        mo[:, 0] = mo[:, 1] = np.arange(M)
+        cr = np.zeros(mo.shape, bool)
+        cr[:, 0] = True
+        add(mo, cr, np.full(M, penalty * (1.0 - 2.0 * n_elec)))
+        add(np.full((1, 4), -1), np.zeros((1, 4), bool), np.array([penalty * n_elec * n_elec]))
+    return np.concatenate(modes), np.concatenate(cre), np.concatenate(coefs)
+
+
+def _jw_strings(modes: np.ndarray, cre: np.ndarray, coefs: np.ndarray, num_sites: int):
+    """Jordan-Wigner images of ladder-operator products as operator strings.
+    a_j = Z x .. x Z x c x I x .. x I, so a product is a tensor product whose
+    site-i factor is the ordered product of Z (operator mode > i), c or c+
+    (mode == i) and I (mode < i).  Each factor is expanded in the letter basis
+    (n and 1-n take two letters, so a term yields up to 2^k strings).  A
+    string is a uint64 key, 2 bits per site, site 0 most significant.
+    Returns (sorted unique keys, summed coefficients), zeros dropped."""
+    L = num_sites
+    if L > 32:
+        raise ValueError("operator strings hold at most 32 sites")
+    T = coefs.size
+    lead = np.zeros((T, L), np.int64)
+    lead_c = np.zeros((T, L))
+    alt = np.full((T, L), -1, np.int64)
+    alt_c = np.zeros((T, L))
+    rows = np.arange(T)
+    for i in range(L):
+        m = np.broadcast_to(I2, (T, 2, 2)).copy()
+        for k in range(modes.shape[1]):
+            ladder = np.where(cre[:, k][:, None, None], _ANN.T, _ANN)
+            f = np.where((modes[:, k] > i)[:, None, None], Z,
+                         np.where((modes[:, k] == i)[:, None, None], ladder, I2))
+            m = m @ f
+        C = np.stack([0.5 * (m[:, 0, 0] + m[:, 1, 1]), 0.5 * (m[:, 0, 0] - m[:, 1, 1]), m[:, 1, 0], m[:, 0, 1]], 1)
+        nz = C != 0.0
+        cnt = nz.sum(1)
+        if cnt.max() > 2:
+            raise AssertionError("a product of ladder operators on one site spans more than two letters")
+        first = np.argmax(nz, 1)
+        lead[:, i], lead_c[:, i] = first, C[rows, first]
+        two = cnt == 2
+        second = 3 - np.argmax(nz[:, ::-1], 1)
+        alt[two, i], alt_c[two, i] = second[two], C[rows[two], second[two]]
+    alive = np.all(lead_c != 0.0, 1)
+    lead, lead_c, alt, alt_c, coefs = lead[alive], lead_c[alive], alt[alive], alt_c[alive], coefs[alive]
+    shifts = (2 * (L - 1 - np.arange(L))).astype(np.uint64)
+    ndouble = (alt >= 0).sum(1)
+    keys_out, c_out = [], []
+    for nd in range(int(ndouble.max(initial=0)) + 1):
+        sel = ndouble == nd
+        if not sel.any():
+            continue
+        ld, lc, al, ac, co = lead[sel], lead_c[sel], alt[sel], alt_c[sel], coefs[sel]
+        r = np.arange(co.size)
+        dpos = np.argsort(al < 0, axis=1, kind="stable")[:, :nd]
+        for combo in range(1 << nd):
+            let, cc = ld.copy(), lc.copy()
+            for b in range(nd):
+                if combo >> b & 1:
+                    col = dpos[:, b]
+                    let[r, col], cc[r, col] = al[r, col], ac[r, col]
+            keys_out.append(np.bitwise_or.reduce(let.astype(np.uint64) << shifts[None, :], axis=1))
+            c_out.append(co * np.prod(cc, 1))
+    keys, inv = np.unique(np.concatenate(keys_out), return_inverse=True)
+    cs = np.bincount(inv.ravel(), weights=np.concatenate(c_out), minlength=keys.size)
+    nz = cs != 0.0
+    return keys[nz], cs[nz]
+
+
+def mpo_from_strings(keys: np.ndarray, coefs: np.ndarray, num_sites: int, tol: float = 1e-12) -> list[np.ndarray]:
+    """Exact MPO of H = sum_s coefs[s] * string_s (keys as from _jw_strings)
+    by the tensor-train SVD of H over the letter basis, run on the sparse
+    representation: at site k the remaining operator is a dense
+    [w * 4] x [distinct suffixes] matrix (row = left bond state, letter at
+    site k), factored by QR of its transpose and an SVD of the triangular
+    factor; singular values below tol * the largest are dropped.  The bond
+    dimensions are the operator-Schmidt ranks of H, as for mpo_from_dense,
+    without forming the 4^L operator."""
+    L = num_sites
+    R = coefs[None, :].astype(np.float64)
+    suffix = keys.astype(np.uint64)
+    factors = []
+    for k in range(L):
+        sh = np.uint64(2 * (L - 1 - k))
+        letter = ((suffix >> sh) & np.uint64(3)).astype(np.int64)
+        rest = suffix & ((np.uint64(1) << sh) - np.uint64(1)) if k < L - 1 else np.zeros_like(suffix)
+        urest, idx = np.unique(rest, return_inverse=True)
+        w = R.shape[0]
+        Mk = np.zeros((w, 4, urest.size))
+        Mk[:, letter, idx.ravel()] = R
+        if k == L - 1:
+            factors.append(Mk.reshape(w, 4, 1))
+            break
+        Mk = Mk.reshape(w * 4, urest.size)
+        if Mk.shape[1] > Mk.shape[0]:
+            U, S, _ = np.linalg.svd(np.linalg.qr(Mk.T, mode="r").T, full_matrices=False)
+        else:
+            U, S, _ = np.linalg.svd(Mk, full_matrices=False)
+        rank = max(1, int((S > tol * S[0]).sum())) if S[0] > 0 else 1
+        U = U[:, :rank]
+        factors.append(U.reshape(w, 4, rank))
+        R = U.T @ Mk
+        suffix = urest
+    # W[a, din, dout, b] = sum_letter F[a, letter, b] * op_letter[dout, din]
+    return [np.ascontiguousarray(np.einsum("aob,oij->ajib", F, _LETTERS)) for F in factors]
+
+
+def qc_mpo_tensors(h: np.ndarray, v: np.ndarray, penalty: float = 0.0, n_elec: int | None = None,
+                   tol: float = 1e-12) -> list[np.ndarray]:
+    """MPO of the operator of qc_hamiltonian_sparse on 2*n_orb sites (n_orb <= 16)
+    from its Jordan-Wigner operator strings, without the dense operator."""
+    n = h.shape[0]
+    ne = n if n_elec is None else n_elec
+    keys, cs = _jw_strings(*_qc_ladder_terms(h, v, penalty, ne), 2 * n)
+    return mpo_from_strings(keys, cs, 2 * n, tol)
+
+
 def qc_tensors(n_orb: int, seed: int = 42, penalty: float = 10.0, n_elec: int | None = None,
-               v_scale: float = 0.1) -> tuple[list[np.ndarray], dict]:
+               v_scale: float = 0.1, exact: bool | None = None) -> tuple[list[np.ndarray], dict]:
     """MPO tensors of the penalised quantum-chemistry Hamiltonian on 2*n_orb
-    sites (exact, by SVD of the dense operator; n_orb <= 6) and its exact
-    ground state: {"E_fci", "E_penalised", "n_elec"} with E_fci the lowest
-    eigenvalue of H in the n_elec sector (half filling by default)."""
-    import scipy.sparse as sp
+    sites (qc_mpo_tensors) and, when exact (default n_orb <= 6), full CI:
+    "E_fci", the lowest eigenvalue of H in the n_elec sector (half filling by
+    default), and for n_orb <= 6 "E_penalised", the lowest eigenvalue of the
+    penalised operator over all sectors."""
     h, v = qc_integrals(n_orb, seed, v_scale)
     ne = n_orb if n_elec is None else n_elec
-    Hs = qc_hamiltonian_sparse(h, v, penalty, ne)
-    Hd = Hs.toarray()
-    M = 2 * n_orb
-    occ = np.array([bin(i).count("1") for i in range(2 ** M)])
-    sector = np.where(occ == ne)[0]
-    H0 = qc_hamiltonian_sparse(h, v, 0.0).toarray()
-    e_fci = float(np.linalg.eigvalsh(H0[np.ix_(sector, sector)])[0])
-    e_pen = float(np.linalg.eigvalsh(Hd)[0])
-    return mpo_from_dense(Hd, M), {"E_fci": e_fci, "E_penalised": e_pen, "n_elec": ne, "h": h, "v": v}
+    sites = qc_mpo_tensors(h, v, penalty, ne)
+    info = {"n_elec": ne, "h": h, "v": v}
+    if exact is None:
+        exact = n_orb <= 6
+    if exact:
+        M = 2 * n_orb
+        occ = np.array([bin(i).count("1") for i in range(2 ** M)])
+        sector = np.where(occ == ne)[0]
+        Hs = qc_hamiltonian_sparse(h, v, 0.0).tocsr()[sector][:, sector]
+        if sector.size <= 4000:
+            info["E_fci"] = float(np.linalg.eigvalsh(Hs.toarray())[0])
+        else:
+            from scipy.sparse.linalg import eigsh
+            info["E_fci"] = float(eigsh(Hs, k=1, which="SA", tol=1e-14)[0][0])
+        if n_orb <= 6:
+            info["E_penalised"] = float(np.linalg.eigvalsh(qc_hamiltonian_sparse(h, v, penalty, ne).toarray())[0])
+    return sites, info

@@ -260,7 +260,9 @@ def _rand_problem(rng, Dl, Dr, d, wL, wM, wR, Dlb=None, Drb=None):
 HEFF_SHAPES = [(1, 2, 2, 1, 5, 1), (2, 4, 2, 5, 5, 5), (7, 9, 2, 5, 5, 5), (16, 16, 2, 8, 8, 8),
                (33, 65, 2, 5, 5, 5), (64, 64, 2, 5, 5, 5), (130, 96, 2, 8, 8, 8), (40, 40, 3, 4, 4, 4),
                # w*d^2 > 64: mixing as a batched GEMM (quantum-chemistry MPO sizes)
-               (20, 24, 2, 40, 40, 40), (17, 9, 2, 100, 70, 30)]
+               (20, 24, 2, 40, 40, 40), (17, 9, 2, 100, 70, 30),
+               # central bonds of the 16-orbital MPO (w = 501, 562, 501)
+               (12, 10, 2, 501, 562, 501)]
 
 
 @pytest.mark.parametrize("shape", HEFF_SHAPES)
@@ -299,7 +301,7 @@ def test_heff_mixed_vs_oracle(ctx, ora, prec, tol, shape):
 
 
 ENV_SHAPES = [(1, 2, 2, 1, 5), (2, 2, 4, 5, 5), (7, 2, 13, 5, 5), (64, 2, 70, 8, 8), (33, 3, 20, 4, 4),
-              (20, 2, 24, 40, 40), (9, 2, 11, 100, 70)]
+              (20, 2, 24, 40, 40), (9, 2, 11, 100, 70), (10, 2, 12, 501, 562)]
 
 
 @pytest.mark.parametrize("shape", ENV_SHAPES)
@@ -413,6 +415,31 @@ def test_lanczos_refined_energy(ctx, ora, prec):
                                                 ct.MPOSite(dev(ctx, Wj)), ctx, 20, 1e-12, prec=prec, refine=1,
                                                 return_info=True)
     assert abs(E_r - E_o) / abs(E_o) < (1e-8 if prec == ct.PREC_FP32 else 1e-6)
+
+
+@pytest.mark.parametrize("prec", [ct.PREC_FP64, ct.PREC_FP32, ct.PREC_BF16X3])
+@pytest.mark.parametrize("refine", [1, 2])
+def test_lanczos_refined_vector(ctx, ora, prec, refine):
+    """Unconverged solve (6 Krylov vectors): the returned vector is normalised,
+    its FP64 Rayleigh quotient is the reported energy (refine 1: exactly; refine 2:
+    up to the Krylov-precision <r|H r> term), and the Rayleigh-Ritz step never
+    raises the energy above refine 1's."""
+    rng = np.random.default_rng(5)
+    L, R, Wi, Wj, th = _sym_problem(rng, 24, 20)
+    out = {}
+    for rf in (1, refine):
+        t = dev(ctx, th)
+        E, v, info = ct.lanczos_two_site_optimize(t, dev(ctx, L), dev(ctx, R), ct.MPOSite(dev(ctx, Wi)),
+                                                  ct.MPOSite(dev(ctx, Wj)), ctx, 6, 1e-12, prec=prec, refine=rf,
+                                                  return_info=True)
+        x = v.to_numpy()
+        rq = float(np.vdot(x, ora.heff_apply(L, Wi, Wj, R, x)) / np.vdot(x, x))
+        out[rf] = (E, rq, float(np.vdot(x, x)))
+    E, rq, nrm = out[refine]
+    assert abs(nrm - 1.0) < 1e-12
+    tol = 1e-12 if (refine == 1 or prec == ct.PREC_FP64) else (1e-9 if prec == ct.PREC_FP32 else 1e-7)
+    assert abs(E - rq) <= tol * abs(rq)
+    assert rq <= out[1][1] + 1e-12 * abs(rq)
 
 
 # ----------------------------------------------------------------- SVD

@@ -94,3 +94,69 @@ def test_qc_mpo_round_trip_and_fci(n_orb):
     assert sites[0].shape[0] == 1 and sites[-1].shape[3] == 1
     # the penalised ground state is the half-filling ground state
     assert abs(info["E_penalised"] - info["E_fci"]) < 1e-10 * max(1.0, abs(info["E_fci"]))
+
+
+def _mpo_apply(sites, x):
+    """H x for an MPO and a state vector (site 0 the most significant bit)."""
+    T = x.reshape([1, -1])  # [bond, remaining input bits then finished output bits]
+    for k, W in enumerate(sites):
+        w = T.shape[0]
+        T = T.reshape(w, 2, -1)  # [a, din_k, rest]
+        T = np.einsum("air,aijb->bjr", T, W)  # [b, dout_k, rest]
+        T = T.reshape(W.shape[3], -1)
+        # move the finished output bit to the back so the next site leads
+        T = T.reshape(W.shape[3], 2, -1).transpose(0, 2, 1).reshape(W.shape[3], -1)
+    return T.reshape(-1)
+
+
+@pytest.mark.parametrize("n_orb,penalty", [(1, 0.0), (2, 0.0), (3, 1.5), (4, 0.7)])
+def test_string_mpo_matches_sparse_operator(n_orb, penalty):
+    h, v = ham.qc_integrals(n_orb, seed=30 + n_orb)
+    sites = ham.qc_mpo_tensors(h, v, penalty, n_orb)
+    H = ham.qc_hamiltonian_sparse(h, v, penalty, n_orb).toarray()
+    assert np.abs(ham.mpo_to_dense(sites) - H).max() < 1e-12 * max(1.0, np.abs(H).max())
+
+
+def test_string_mpo_has_optimal_bond_dimensions():
+    """The sparse tensor-train SVD finds the operator-Schmidt ranks, i.e. the
+    bond dimensions of the SVD of the dense operator."""
+    h, v = ham.qc_integrals(5, seed=4)
+    sites = ham.qc_mpo_tensors(h, v, 1.0, 5)
+    dense = ham.mpo_from_dense(ham.qc_hamiltonian_sparse(h, v, 1.0, 5).toarray(), 10)
+    assert [W.shape[3] for W in sites] == [W.shape[3] for W in dense]
+
+
+def test_string_mpo_apply_matches_sparse_six_orbitals():
+    h, v = ham.qc_integrals(6, seed=9)
+    sites = ham.qc_mpo_tensors(h, v, 1.0, 6)
+    H = ham.qc_hamiltonian_sparse(h, v, 1.0, 6)
+    rng = np.random.default_rng(0)
+    for _ in range(2):
+        x = rng.standard_normal(4 ** 6)
+        ref = H @ x
+        assert np.abs(_mpo_apply(sites, x) - ref).max() < 1e-11 * np.abs(ref).max()
+
+
+def test_string_mpo_center_rank_is_complementary_operator_count():
+    """For generic integrals the central operator-Schmidt rank over k = 2n/2
+    spin-orbitals per side is 1 (identity) + 1 (complete left block)
+    + 2k (single ladder operators) + 2k (their right complements)
+    + k(k-1) (pairs a+a+ and aa) + k^2 (pairs a+a) (Chan et al.,
+    J. Chem. Phys. 145, 014102 (2016))."""
+    n = 8
+    h, v = ham.qc_integrals(n, seed=2)
+    sites = ham.qc_mpo_tensors(h, v, 1.0, n)
+    k = n
+    assert sites[k - 1].shape[3] == 2 + 4 * k + k * (k - 1) + k * k
+    assert [W.shape[3] for W in sites[:-1]] == [W.shape[3] for W in sites[:-1]][::-1]
+
+
+def test_penalty_only_mpo_rank():
+    """mu (N - n)^2 alone: left operators I, N_L, N_L^2 -> rank <= 3."""
+    n = 4
+    sites = ham.qc_mpo_tensors(np.zeros((n, n)), np.zeros((n,) * 4), 2.0, 3)
+    assert max(W.shape[3] for W in sites[:-1]) <= 3
+    M = 2 * n
+    N = np.diag([bin(s).count("1") for s in range(2 ** M)]).astype(float)
+    ref = 2.0 * (N - 3 * np.eye(2 ** M)) @ (N - 3 * np.eye(2 ** M))
+    assert np.abs(ham.mpo_to_dense(sites) - ref).max() < 1e-12 * np.abs(ref).max()
