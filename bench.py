@@ -116,6 +116,37 @@ def bench_matvec(ctx, D, w, steps, warmup):
     return res, F
 
 
+def bench_matvec_mpo(ctx, D, Wi, Wj, steps, warmup, seed=7):
+    """Prepared-operator matvec of given MPO tensors at bond dimension D (random symmetric environments), three
+    forms; returns {form: {"s_per_matvec", "tflops"}} and the relative errors against FP64."""
+    import torch
+    import chemtensor_mojo_amd as ct
+    rng = np.random.default_rng(seed)
+    wL, wR = Wi.shape[0], Wj.shape[3]
+    L = rng.standard_normal((wL, D, D)) / np.sqrt(D)
+    L = 0.5 * (L + L.transpose(0, 2, 1))
+    R = rng.standard_normal((wR, D, D)) / np.sqrt(D)
+    R = 0.5 * (R + R.transpose(0, 2, 1))
+    th = rng.standard_normal((D, 2, 2, D))
+    th /= np.linalg.norm(th)
+    dev = lambda a: ct.create_dense_tensor_from_data(ctx, a, a.shape)
+    tL, tR, tth = dev(L), dev(R), dev(th)
+    F = matvec_flops(D, 2, D, wL, Wi.shape[3], wR)
+    res, outs = {}, {}
+    for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3)):
+        op = ct.HeffOperator(tL, tR, ct.MPOSite(dev(Wi)), ct.MPOSite(dev(Wj)), ctx, prec=prec)
+        buf = torch.empty(D * 4 * D, dtype=torch.float64, device="cuda")
+        t, _ = time_events(lambda: op.apply(tth, buf), steps, warmup)
+        outs[name] = buf.cpu().numpy()
+        op.close()
+        res[name] = {"s_per_matvec": t, "tflops": F / t / 1e12}
+    for name in ("fp32", "bf16x3"):
+        res[name]["rel_err_vs_fp64"] = float(np.linalg.norm(outs[name] - outs["fp64"]) / np.linalg.norm(outs["fp64"]))
+    del tL, tR, tth
+    torch.cuda.empty_cache()
+    return res, F
+
+
 def bench_matvec_grid(ctx, steps, warmup, Ds=(256, 512, 1024), ws=(5, 8)):
     """Prepared-operator matvec rate (TFLOP/s, mean over `steps` calls timed with HIP events) over D and w: the
     microbenchmark grid (XXZ bulk MPO for w=5, J1-J2 bulk MPO for w=8)."""
@@ -512,6 +543,10 @@ def study_qc16(ctx, args):
     pen_t = ham.qc_mpo_tensors(np.zeros_like(h), np.zeros_like(v), 1.0, n_orb)
     log(f"QC MPO built in {time.perf_counter() - t0:.1f} s; bond dims {[W.shape[3] for W in mpo_t[:-1]]}")
     L = 2 * n_orb
+    center, Fc = bench_matvec_mpo(ctx, D, mpo_t[L // 2 - 1], mpo_t[L // 2], 5, 2)
+    for k, r in center.items():
+        log(f"center-bond matvec {k}: {r['s_per_matvec'] * 1e3:.2f} ms {r['tflops']:.1f} TFLOP/s "
+            f"(err vs fp64 {r.get('rel_err_vs_fp64', 0.0):.1e})")
     mpo = ct.mpo_from_tensors(ctx, mpo_t)
     forms = (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2), ("bf16x3_refined", ct.PREC_BF16X3, 2))
 
@@ -554,6 +589,8 @@ def study_qc16(ctx, args):
             f"vs fp64 refined {warm[form]['dE_rel_vs_fp64_refined']:.2e} {warm[form]['s']:.1f} s")
     print(json.dumps({"study": "qc16", "workload": study_qc16.__doc__.replace("\n", " "),
                       "mpo_bond_dims": [int(W.shape[3]) for W in mpo_t[:-1]],
+                      "center_bond_matvec": {"D": D, "w": [int(mpo_t[L // 2 - 1].shape[0]), int(mpo_t[L // 2].shape[0]),
+                                                           int(mpo_t[L // 2].shape[3])], "flops": Fc, **center},
                       "summary": {k: {kk: vv for kk, vv in v.items() if kk != "sweeps"} for k, v in out.items()},
                       "warm_sweep": warm, "sweeps": {k: v["sweeps"] for k, v in out.items()}}), flush=True)
 

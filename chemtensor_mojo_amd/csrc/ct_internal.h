@@ -109,7 +109,8 @@ enum WsSlot {
     WS_EIG5 = 31,    // back-transformation: V^T V and T per block
     WS_MIXA = 32,    // large-w mixing: packed input
     WS_MIXB = 33,    // large-w mixing: GEMM output
-    WS_COUNT = 34
+    WS_MIXC = 34,    // large-w mixing: split coefficients (3xBF16)
+    WS_COUNT = 35
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);
@@ -150,6 +151,10 @@ inline int64_t split_k32(int64_t K) { return (K + 31) / 32 * 32; }
 // split rows at out (ld_out, bs_out), zero-filling k in [K, split_k32(K)).
 int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
                int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active);
+// Same with rows contiguous (s_row == 1) and a two-level k: element k of a row
+// at X + (k / kin)*s_kc + (k % kin)*s_k.
+int split_bf16_kk(Handle* h, int tX, const void* X, int rows, int K, int kin, int64_t s_k, int64_t s_kc, int batch,
+                  int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active);
 // Zeroes k in [K, split_k32(K)) of every row of a split-rows operand.
 int split_zero_pad(Handle* h, uint16_t* out, int64_t rows, int64_t K, int64_t ld);
 // C[b][m][n] = alpha * sum_k (Ah Bh + Ah Bl + Al Bh)[m,k][n,k] + beta * C

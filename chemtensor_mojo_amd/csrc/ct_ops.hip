@@ -309,6 +309,29 @@ int mix_large(Handle* h, const TX* X, void* Y, const double* d_coef, const MixAr
     return 0;
 }
 
+// 3xBF16 form with split-rows output: the coefficients (rows n) and X (rows q
+// of each pixel row p, k = (ci, ei)) are split once into bf16 hi/lo rows and
+// the three-product NT GEMM forms Yt[p][n][q] in FP32 before the unpack.
+int mix_large_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, const MixArgs& a) {
+    const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
+    const int64_t ld = 2 * split_k32(K);
+    void *wx, *wy, *wc;
+    CT_CHECK(ws_get(h, WS_MIXA, (size_t)(a.P * a.Q * ld) * 2, &wx));
+    CT_CHECK(ws_get(h, WS_MIXB, (size_t)(a.P * NO * a.Q) * 4, &wy));
+    CT_CHECK(ws_get(h, WS_MIXC, (size_t)(NO * ld) * 2, &wc));
+    CT_CHECK(split_bf16(h, CT_F64, d_coef, NO, K, 1, NO, 1, 0, (uint16_t*)wc, ld, 0, a.active));
+    CT_CHECK(split_bf16_kk(h, CT_F32, X, (int)a.Q, K, a.EIN, a.sXe, a.sXc, (int)a.P, a.sXp, (uint16_t*)wx, ld,
+                           a.Q * ld, a.active));
+    const SplitOp A{(const uint16_t*)wc, ld, 0}, B{(const uint16_t*)wx, ld, a.Q * ld};
+    CT_CHECK(gemm_bf16x3_nt(h, A, B, CT_F32, wy, NO, (int)a.Q, K, a.Q, (int64_t)NO * a.Q, (int)a.P, 1.0, 0.0,
+                            a.active));
+    const int64_t nout = a.P * (int64_t)NO * a.Q;
+    hipLaunchKernelGGL((mix_unpack_kernel<float, 2>), dim3((unsigned)std::min<int64_t>((nout + 255) / 256, 16384)),
+                       dim3(256), 0, h->stream, (const float*)wy, (void*)Y, a);
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
 bool mix_is_large(const MixArgs& a) { return a.CIN * a.EIN > MIX_KMAX || a.COUT * a.EOUT > MIX_NMAX; }
 
 }  // namespace
@@ -366,7 +389,7 @@ int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, cons
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
     CT_CHECK(mix_args(m, &a));
-    if (mix_is_large(a)) return mix_large<float, float, 2>(h, X, (void*)Y, d_coef, a);
+    if (mix_is_large(a)) return mix_large_split(h, X, Y, d_coef, a);
     CT_CHECK((launch_mix<float, float, 2>(h, X, (void*)Y, d_coef, a, padded)));
     CT_HIP(hipGetLastError());
     return 0;
