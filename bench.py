@@ -595,6 +595,20 @@ def study_qc16(ctx, args):
                       "warm_sweep": warm, "sweeps": {k: v["sweeps"] for k, v in out.items()}}), flush=True)
 
 
+def study_qc16_matvec(ctx, args):
+    """The H_eff matvec at the centre bond of the 16-orbital quantum-chemistry MPO (w = 501, 562, 501; d = 2), D = 512,
+    random symmetric environments, prepared operator, FP64 / FP32 / 3xBF16 (`--steps` timed calls each); used for
+    the kernel profile of the large-w path."""
+    from chemtensor_mojo_amd import hamiltonians as ham
+    h, v = ham.qc_integrals(16, 42)
+    mpo_t = ham.qc_mpo_tensors(h, v, 1.0, 16)
+    res, F = bench_matvec_mpo(ctx, 512, mpo_t[15], mpo_t[16], args.steps, args.warmup)
+    for k, r in res.items():
+        log(f"centre-bond matvec {k}: {r['s_per_matvec'] * 1e3:.2f} ms {r['tflops']:.1f} TFLOP/s")
+    print(json.dumps({"study": "qc16_matvec", "workload": study_qc16_matvec.__doc__.replace("\n", " "), "flops": F,
+                      "by_form": res}), flush=True)
+
+
 def study_eigensolver(ctx, args):
     """The Gram-split eigensolver alone at n = 2048 (the two-site split at D = 1024, d = 2), ours against rocSOLVER
     dsyevd; used for the kernel profile of the eigensolver."""
@@ -634,7 +648,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--qc-sweeps", type=int, default=6)
     ap.add_argument("--qc-penalty", type=float, default=1.0)
-    ap.add_argument("--study", choices=("stages", "precision", "adjoint", "eigensolver", "qc", "qc16"), default=None,
+    ap.add_argument("--study", choices=("stages", "precision", "adjoint", "eigensolver", "qc", "qc16",
+                                            "qc16_matvec"), default=None,
                     help="run one of the larger studies instead of the default benchmark")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -649,7 +664,8 @@ def main():
     log(f"device: {torch.cuda.get_device_name(0)}")
     if args.study:
         {"stages": study_stages, "precision": study_precision, "adjoint": study_adjoint,
-         "eigensolver": study_eigensolver, "qc": study_qc, "qc16": study_qc16}[args.study](ctx, args)
+         "eigensolver": study_eigensolver, "qc": study_qc, "qc16": study_qc16,
+         "qc16_matvec": study_qc16_matvec}[args.study](ctx, args)
         ctx.close()
         return
     mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)
