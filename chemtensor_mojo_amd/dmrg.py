@@ -22,7 +22,7 @@ from .state import MatrixProductOperator, MatrixProductState, MPOSite, MPSSite
 
 @dataclass
 class DMRGParams:
-    """DMRGParams (dmrg.mojo:22-36) plus the precision controls of this build.
+    """DMRGParams (dmrg.mojo:22-36) plus precision controls without a reference counterpart.
 
     prec            precision of the Krylov matvecs (PREC_FP64 / PREC_FP32 / PREC_BF16X3)
     refine          0: reference energies; 1: FP64 Rayleigh quotient of each local
