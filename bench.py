@@ -529,8 +529,9 @@ def _mpo_expectation(mps_sites, mpo_sites):
 def study_qc16(ctx, args):
     """Quantum-chemistry-shaped DMRG at full size: n_orb = 16 spatial orbitals (32 spin-orbital sites, Jordan-Wigner),
     h_pq symmetric N(0,1), (pq|rs) N(0,0.1) with 8-fold symmetry (seed 42), half filling (16 electrons) through the
-    particle-number penalty 1.0 (N - 16)^2, exact MPO from the operator strings (central bond dimension 562),
-    PCG32 random start D=16, chi_max = 512, eps_trunc = 1e-12, 6 sweeps, 20 Krylov vectors, exact tridiagonal solves;
+    particle-number penalty 1.0 (N - 16)^2, exact MPO from the operator strings (central bond dimension 562), seeded
+    random start D=16 inside the 16-electron sector (hamiltonians.number_sector_random_mps_tensors; H conserves N, so
+    the sweeps stay there), chi_max = 512, eps_trunc = 1e-12, 6 sweeps, 20 Krylov vectors, exact tridiagonal solves;
     FP64 against FP32 and 3xBF16 with FP64 refinement.  Then one more sweep in each form, and in FP64 with the same
     refinement, from the same FP64 state (isolates the arithmetic from the different trajectories of truncated runs),
     and <(N - 16)^2> of each result."""
@@ -556,7 +557,8 @@ def study_qc16(ctx, args):
 
     out, warm, fp64_state = {}, {}, None
     for form, prec, refine in forms:
-        run = ct.DMRGRun(ctx, mpo, ct.create_random_mps_c_compatible(ctx, L, 2, 16, 42), params(prec, refine))
+        start = ct.mps_from_tensors(ctx, ham.number_sector_random_mps_tensors(L, n_orb, 16, 42))
+        run = ct.DMRGRun(ctx, mpo, start, params(prec, refine))
         rows = _sweep_deltas(run, sweeps, form)
         state = run.mps()
         run.close()

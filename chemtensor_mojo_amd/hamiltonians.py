@@ -426,3 +426,28 @@ def qc_tensors(n_orb: int, seed: int = 42, penalty: float = 10.0, n_elec: int | 
         if n_orb <= 6:
             info["E_penalised"] = float(np.linalg.eigvalsh(qc_hamiltonian_sparse(h, v, penalty, ne).toarray())[0])
     return sites, info
+
+
+def number_sector_random_mps_tensors(num_sites: int, n_elec: int, D: int, seed: int = 42) -> list[np.ndarray]:
+    """Random MPS [Dl, 2, Dr] of definite particle number n_elec: every bond
+    index carries a left electron count q (0 <= q <= n_elec, reachable from both
+    ends), the bond's indices cycle through the reachable counts nearest the
+    mean filling, and A[l, s, r] ~ N(0,1) only where q_r = q_l + s.  The state
+    lies entirely in the n_elec sector; N-conserving Hamiltonians keep it there."""
+    rng = np.random.default_rng(seed)
+    L = num_sites
+    charges = []
+    for k in range(L + 1):
+        lo, hi = max(0, n_elec - (L - k)), min(k, n_elec)
+        feasible = sorted(range(lo, hi + 1), key=lambda q: (abs(q - k * n_elec / L), q))
+        Dk = min(D, 2 ** min(k, L - k))
+        charges.append(np.array([feasible[i % len(feasible)] for i in range(Dk)]))
+    sites = []
+    for k in range(L):
+        ql, qr = charges[k], charges[k + 1]
+        A = np.zeros((ql.size, 2, qr.size))
+        for s in (0, 1):
+            mask = (ql[:, None] + s) == qr[None, :]
+            A[:, s, :] = np.where(mask, rng.standard_normal(mask.shape), 0.0)
+        sites.append(A / max(np.linalg.norm(A), 1e-300))
+    return sites

@@ -160,3 +160,16 @@ def test_penalty_only_mpo_rank():
     N = np.diag([bin(s).count("1") for s in range(2 ** M)]).astype(float)
     ref = 2.0 * (N - 3 * np.eye(2 ** M)) @ (N - 3 * np.eye(2 ** M))
     assert np.abs(ham.mpo_to_dense(sites) - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_number_sector_random_mps():
+    L, ne = 10, 5
+    sites = ham.number_sector_random_mps_tensors(L, ne, 8, seed=3)
+    psi = sites[0]
+    for A in sites[1:]:
+        psi = np.tensordot(psi, A, axes=([-1], [0]))
+    psi = psi.reshape(-1)
+    occ = np.array([bin(i).count("1") for i in range(2 ** L)])
+    assert np.linalg.norm(psi) > 0
+    assert np.abs(psi[occ != ne]).max() == 0.0
+    assert max(A.shape[2] for A in sites) == 8
