@@ -56,8 +56,7 @@ __device__ __forceinline__ double block_sum256(double v) {
 // tiles, then walked in groups of GM tile rows (column-major inside a group),
 // so the A row-panels and B column-panels one XCD touches overlap and stay in
 // its L2.
-__device__ __forceinline__ void gemm_tile_coords(int bid, int nb, int tilesM, int tilesN, int GM, int& tm, int& tn) {
-    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+__device__ __forceinline__ void gemm_group_coords(int bid, int tilesM, int tilesN, int GM, int& tm, int& tn) {
     const int group = bid / (GM * tilesN);
     const int first = group * GM;
     const int gm = min(tilesM - first, GM);
@@ -66,8 +65,25 @@ __device__ __forceinline__ void gemm_tile_coords(int bid, int nb, int tilesM, in
     tn = in / gm;
 }
 
+__device__ __forceinline__ void gemm_tile_coords(int bid, int nb, int tilesM, int tilesN, int GM, int& tm, int& tn) {
+    if ((nb & 7) == 0) bid = (bid & 7) * (nb >> 3) + (bid >> 3);
+    gemm_group_coords(bid, tilesM, tilesN, GM, tm, tn);
+}
+
 __device__ __forceinline__ void gemm_tile_coords(int bid, int nb, int tilesM, int tilesN, int& tm, int& tn) {
     gemm_tile_coords(bid, nb, tilesM, tilesN, 8, tm, tn);
+}
+
+// Batched launches with few tiles per batch (grid = tiles x 1 x batch, batch a
+// multiple of 8): workgroup f = z*nb + x runs on XCD f mod 8, so give XCD c
+// whole batches c, c+8, ... and walk their tiles in order.  Every batch's
+// operand panels then leave HBM once instead of once per XCD.  A bijection of
+// (x, z) onto (batch, tile).
+__device__ __forceinline__ void gemm_batch_per_xcd(int nb, int& b, int& tile) {
+    const int f = (int)blockIdx.z * nb + (int)blockIdx.x;
+    const int s = f >> 3;
+    b = (f & 7) + 8 * (s / nb);
+    tile = s % nb;
 }
 
 }  // namespace ct

@@ -80,11 +80,6 @@ __device__ __forceinline__ void load4<float, float>(const float* p, float* o) {
     v4f a = *(const v4f*)p;
     o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
 }
-template <>
-__device__ __forceinline__ void load4<float, double>(const float* p, double* o) {
-    v4f a = *(const v4f*)p;
-    o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-}
 
 // Tile loader.  KC == false: operand X[k][x] (x contiguous, row stride sk);
 // KC == true: operand X[x][k] (k contiguous, row stride sx).  Fills 8

@@ -157,6 +157,7 @@ struct NtArgs {
     const int* active;
     int GM;                 // tile rows per L2 group (see gemm_tile_coords)
     int splits;             // split-K factor
+    int batch_xcd;          // 256x256 kernel: whole batches per XCD (gemm_batch_per_xcd)
     int kchunk;             // K range per split, multiple of 32
     float* part;            // [batch*splits][M][N] FP32 partial sums when splits > 1
 };
@@ -184,11 +185,20 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_dma_kernel(NtArgs p) {
     if (p.active && *p.active == 0) return;
     __shared__ __attribute__((aligned(1024))) char smem[2 * STAGE];
 
-    int tm, tn;
-    gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, p.GM, tm, tn);
+    int tm, tn, b, sk, z;
+    if (p.batch_xcd) {  // splits == 1
+        int tile;
+        gemm_batch_per_xcd(gridDim.x, b, tile);
+        gemm_group_coords(tile, p.tilesM, p.tilesN, p.GM, tm, tn);
+        sk = 0;
+        z = b;
+    } else {
+        gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, p.GM, tm, tn);
+        z = blockIdx.z;
+        b = z / p.splits;
+        sk = z - b * p.splits;
+    }
     const int m0 = tm * BB, n0 = tn * BB;
-    const int z = blockIdx.z;
-    const int b = z / p.splits, sk = z - b * p.splits;
     const int kbeg = sk * p.kchunk;
     const int kend = min(p.K32, kbeg + p.kchunk);
 
@@ -503,6 +513,7 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
     p.alpha = alpha; p.beta = beta; p.active = active;
     p.GM = 8;
     p.splits = 1;
+    p.batch_xcd = 0;
     p.kchunk = K32;
     p.part = nullptr;
     if (K == 0) {  // empty contraction: C = beta * C
@@ -524,6 +535,11 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
         if (p.GM < 1) p.GM = 1;
         p.splits = splits;
         p.kchunk = (K32 / splits + BK - 1) / BK * BK;
+        // few tiles per batch: keep each batch on one XCD so its panels are fetched once
+        if (splits == 1 && batch % 8 == 0 && p.tilesM * p.tilesN <= 32) {
+            p.batch_xcd = 1;
+            p.GM = p.tilesM;
+        }
         dim3 grid(p.tilesM * p.tilesN, 1, batch * splits), block(256);
         if (splits > 1) {
             void* w;
