@@ -109,9 +109,9 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ X, int
 // 32 (k) x 64 (rows) tile goes through LDS; each thread then splits 8
 // consecutive k of one row and writes their hi and lo halves as one 16-byte
 // store each (4 threads fill a row's 128-byte line).  Needs ld % 8 == 0 and a
-// 16-byte aligned output base.  Element k of a row sits at
-// (k / kin) * s_kc + (k % kin) * s_k (kin = K for a single k stride).
-template <typename T>
+// 16-byte aligned output base.  Element k of a row sits at k * s_k, or with
+// TWO_LEVEL at (k / kin) * s_kc + (k % kin) * s_k.
+template <typename T, bool TWO_LEVEL>
 __global__ __launch_bounds__(256) void split_t_kernel(const T* __restrict__ X, int rows, int K, int kin, int64_t s_k,
                                                       int64_t s_kc, int64_t bsX, uint16_t* __restrict__ out,
                                                       int64_t ld, int64_t bs, const int* active) {
@@ -125,8 +125,8 @@ __global__ __launch_bounds__(256) void split_t_kernel(const T* __restrict__ X, i
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int kk = (t >> 6) + 4 * i, k = k0 + kk, r = r0 + (t & 63);
-        tile[kk][t & 63] =
-            (k < K && r < rows) ? (float)X[(int64_t)r + (int64_t)(k / kin) * s_kc + (int64_t)(k % kin) * s_k] : 0.f;
+        const int64_t ko = TWO_LEVEL ? (int64_t)(k / kin) * s_kc + (int64_t)(k % kin) * s_k : (int64_t)k * s_k;
+        tile[kk][t & 63] = (k < K && r < rows) ? (float)X[(int64_t)r + ko] : 0.f;
     }
     __syncthreads();
     const int row = t >> 2, grp = t & 3;
@@ -455,11 +455,11 @@ int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row,
     if (s_row == 1 && ld_out % 8 == 0 && bs_out % 8 == 0 && ((uintptr_t)out % 16) == 0) {
         dim3 grid((unsigned)((rows + 63) / 64), (unsigned)(split_k32(K) / 32), (unsigned)batch), block(256);
         if (tX == CT_F64)
-            hipLaunchKernelGGL(split_t_kernel<double>, grid, block, 0, h->stream, (const double*)X, rows, K, K, s_k,
-                               (int64_t)0, bsX, out, ld_out, bs_out, active);
+            hipLaunchKernelGGL((split_t_kernel<double, false>), grid, block, 0, h->stream, (const double*)X, rows, K,
+                               K, s_k, (int64_t)0, bsX, out, ld_out, bs_out, active);
         else
-            hipLaunchKernelGGL(split_t_kernel<float>, grid, block, 0, h->stream, (const float*)X, rows, K, K, s_k,
-                               (int64_t)0, bsX, out, ld_out, bs_out, active);
+            hipLaunchKernelGGL((split_t_kernel<float, false>), grid, block, 0, h->stream, (const float*)X, rows, K, K,
+                               s_k, (int64_t)0, bsX, out, ld_out, bs_out, active);
         CT_HIP(hipGetLastError());
         return 0;
     }
@@ -484,11 +484,11 @@ int split_bf16_kk(Handle* h, int tX, const void* X, int rows, int K, int kin, in
     }
     dim3 grid((unsigned)((rows + 63) / 64), (unsigned)(split_k32(K) / 32), (unsigned)batch), block(256);
     if (tX == CT_F64)
-        hipLaunchKernelGGL(split_t_kernel<double>, grid, block, 0, h->stream, (const double*)X, rows, K, kin, s_k,
-                           s_kc, bsX, out, ld_out, bs_out, active);
+        hipLaunchKernelGGL((split_t_kernel<double, true>), grid, block, 0, h->stream, (const double*)X, rows, K, kin,
+                           s_k, s_kc, bsX, out, ld_out, bs_out, active);
     else
-        hipLaunchKernelGGL(split_t_kernel<float>, grid, block, 0, h->stream, (const float*)X, rows, K, kin, s_k, s_kc,
-                           bsX, out, ld_out, bs_out, active);
+        hipLaunchKernelGGL((split_t_kernel<float, true>), grid, block, 0, h->stream, (const float*)X, rows, K, kin,
+                           s_k, s_kc, bsX, out, ld_out, bs_out, active);
     CT_HIP(hipGetLastError());
     return 0;
 }

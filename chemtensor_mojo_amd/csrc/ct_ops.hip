@@ -42,7 +42,7 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 // per-element guards; every coefficient address is a compile-time offset
 // from a kernel argument, so the FMAs take them from SGPRs loaded through
 // the scalar cache.
-template <typename TX, typename TACC, int OUT, int KM>
+template <typename TX, typename TACC, int OUT, int KM, bool FULL>
 __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void* __restrict__ Yv,
                                                   const TACC* __restrict__ coef, MixArgs a) {
     if (a.active && *a.active == 0) return;
@@ -81,9 +81,10 @@ __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void
         for (int k = 0; k < KM; ++k)
 #pragma unroll
             for (int n = 0; n < KM; ++n) acc[n] += coef[k * KM + n] * x[k];
+        // FULL (NO == KM): no guard, so the loop stays unrolled
 #pragma unroll
         for (int n = 0; n < KM; ++n) {
-            if (n >= NO) break;
+            if (!FULL && n >= NO) break;
             if (OUT == 0) {
                 ((double*)Yv)[out_off[n] + p * a.sYp + q] = (double)acc[n];
             } else if (OUT == 1) {
@@ -229,12 +230,26 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
         c = (const TACC*)w;
     }
     dim3 grid((unsigned)((a.Q + 255) / 256), (unsigned)(a.P < 65535 ? a.P : 65535)), block(256);
+    // FP64 with every output channel real: the unguarded store loop keeps acc[] in
+    // registers (the guarded loop is rolled and puts acc[] in scratch).  For FP32 the
+    // guarded loop stays in registers at higher occupancy and measures faster.
+    if constexpr (sizeof(TACC) == 8) {
+        if (NO == KM) {
+            if (KM == 16)
+                hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16, true>), grid, block, 0, h->stream, X, Y, c, a);
+            else if (KM == 32)
+                hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32, true>), grid, block, 0, h->stream, X, Y, c, a);
+            else
+                hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 64, true>), grid, block, 0, h->stream, X, Y, c, a);
+            return 0;
+        }
+    }
     if (KM == 16)
-        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16>), grid, block, 0, h->stream, X, Y, c, a);
+        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16, false>), grid, block, 0, h->stream, X, Y, c, a);
     else if (KM == 32)
-        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32>), grid, block, 0, h->stream, X, Y, c, a);
+        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32, false>), grid, block, 0, h->stream, X, Y, c, a);
     else
-        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 64>), grid, block, 0, h->stream, X, Y, c, a);
+        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 64, false>), grid, block, 0, h->stream, X, Y, c, a);
     return 0;
 }
 
@@ -242,8 +257,9 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
 // When w*d (environments) or w*d^2 (H_eff) exceeds 64 the per-pixel mixing
 // is a real matrix product: per pixel row p, Y_p[n][q] = sum_k coef[k][n] X_p[k][q]
 // with K, NO up to thousands.  X is packed to [P][K][Q], one batched MFMA
-// GEMM (FP64, or FP32 for FP32 / 3xBF16 contractions) forms [P][NO][Q], and an
-// unpack kernel writes Y's layout (plain, or 3xBF16 split rows).
+// GEMM (FP64, or FP32 for FP32 contractions) forms [P][NO][Q], and an unpack
+// kernel writes Y's layout.  The 3xBF16 form (mix_large_split) splits both
+// operands and runs the three-product BF16 GEMM instead.
 template <typename TX, typename TT>
 __global__ void mix_pack_kernel(const TX* __restrict__ X, TT* __restrict__ Xt, MixArgs a) {
     if (a.active && *a.active == 0) return;
