@@ -647,6 +647,7 @@ def main():
     ap.add_argument("--sweep-D", type=int, default=1024)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the eigensolver and matvec-grid sections")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--qc-sweeps", type=int, default=6)
     ap.add_argument("--qc-penalty", type=float, default=1.0)
@@ -672,8 +673,8 @@ def main():
         return
     mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)
     gm, Fg = bench_dominant_gemm(ctx, D, w, args.steps, args.warmup)
-    eig = bench_eigensolver(ctx)
-    mgrid = bench_matvec_grid(ctx, max(5, args.steps // 2), args.warmup)
+    eig = None if args.no_extras else bench_eigensolver(ctx)
+    mgrid = None if args.no_extras else bench_matvec_grid(ctx, max(5, args.steps // 2), args.warmup)
     sweep = None
     if not args.no_sweep:
         s64 = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_FP64, 0)
