@@ -36,7 +36,7 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 }
 
 // OUT: 0 = double, 1 = float, 2 = split BF16 hi/lo planes (uint16).
-// KM: compile-time size (16, 32 or 64) of the zero-padded KM x KM coefficient
+// KM: compile-time size (16, 20, 32 or 64) of the zero-padded KM x KM coefficient
 // block `coef` (row k = input, column n = output).  Inputs k >= K re-read
 // input K-1 and meet zero coefficients, so the unrolled loops carry no
 // per-element guards; every coefficient address is a compile-time offset
@@ -213,7 +213,11 @@ int mix_args(const MixDesc& m, MixArgs* a) {
     return 0;
 }
 
-static int mix_km(int K, int NO) { return (K <= 16 && NO <= 16) ? 16 : (K <= 32 && NO <= 32) ? 32 : 64; }
+// Padded block size: 20 fits the reference's w = 5 MPOs at d = 2 exactly (K = N_O = 20).
+static int mix_km(int K, int NO) {
+    const int m = K > NO ? K : NO;
+    return m <= 16 ? 16 : m <= 20 ? 20 : m <= 32 ? 32 : 64;
+}
 static int mix_km(const MixArgs& a) { return mix_km(a.CIN * a.EIN, a.COUT * a.EOUT); }
 
 // Padded coefficient block (workspace, unless `padded` already holds it) in
@@ -237,6 +241,8 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
         if (NO == KM) {
             if (KM == 16)
                 hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16, true>), grid, block, 0, h->stream, X, Y, c, a);
+            else if (KM == 20)
+                hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 20, true>), grid, block, 0, h->stream, X, Y, c, a);
             else if (KM == 32)
                 hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32, true>), grid, block, 0, h->stream, X, Y, c, a);
             else
@@ -246,6 +252,8 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
     }
     if (KM == 16)
         hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16, false>), grid, block, 0, h->stream, X, Y, c, a);
+    else if (KM == 20)
+        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 20, false>), grid, block, 0, h->stream, X, Y, c, a);
     else if (KM == 32)
         hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32, false>), grid, block, 0, h->stream, X, Y, c, a);
     else
