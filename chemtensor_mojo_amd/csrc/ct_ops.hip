@@ -309,6 +309,35 @@ __global__ void mix_unpack_kernel(const TT* __restrict__ Yt, void* __restrict__ 
     }
 }
 
+// Split-rows unpack with four neighbouring q per thread (Q, sYc, sYe, sYp
+// multiples of 4): each thread writes its four hi and four lo BF16 values as
+// one 8-byte store each.
+__global__ void mix_unpack_split4_kernel(const float* __restrict__ Yt, uint16_t* __restrict__ Y, MixArgs a) {
+    if (a.active && *a.active == 0) return;
+    const int NO = a.COUT * a.EOUT;
+    const int64_t Q4 = a.Q >> 2;
+    const int64_t total = a.P * (int64_t)NO * Q4;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = (t % Q4) << 2, pn = t / Q4;
+        const int n = (int)(pn % NO);
+        const int64_t p = pn / NO;
+        const int co = n / a.EOUT, eo = n - co * a.EOUT;
+        const float4 v = *(const float4*)(Yt + pn * a.Q + q);
+        const int64_t k = co * a.sYc + q;  // k % 4 == 0: the four k share a 32-block
+        const int64_t o = eo * a.sYe + p * a.sYp + 2 * (k & ~(int64_t)31) + (k & 31);
+        const float f[4] = {v.x, v.y, v.z, v.w};
+        uint16_t hi[4], lo[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            hi[i] = f2bf(f[i]);
+            lo[i] = f2bf(f[i] - __uint_as_float(((uint32_t)hi[i]) << 16));
+        }
+        *(uint2*)(Y + o) = make_uint2((uint32_t)hi[0] | ((uint32_t)hi[1] << 16), (uint32_t)hi[2] | ((uint32_t)hi[3] << 16));
+        *(uint2*)(Y + o + 32) =
+            make_uint2((uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16));
+    }
+}
+
 template <typename TX, typename TT, int OUT>
 int mix_large(Handle* h, const TX* X, void* Y, const double* d_coef, const MixArgs& a) {
     const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
@@ -350,8 +379,13 @@ int mix_large_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef
     CT_CHECK(gemm_bf16x3_nt(h, A, B, CT_F32, wy, NO, (int)a.Q, K, a.Q, (int64_t)NO * a.Q, (int)a.P, 1.0, 0.0,
                             a.active));
     const int64_t nout = a.P * (int64_t)NO * a.Q;
-    hipLaunchKernelGGL((mix_unpack_kernel<float, 2>), dim3((unsigned)std::min<int64_t>((nout + 255) / 256, 16384)),
-                       dim3(256), 0, h->stream, (const float*)wy, (void*)Y, a);
+    if (a.Q % 4 == 0 && a.sYc % 4 == 0 && a.sYe % 4 == 0 && a.sYp % 4 == 0 && ((uintptr_t)Y % 8) == 0)
+        hipLaunchKernelGGL(mix_unpack_split4_kernel,
+                           dim3((unsigned)std::min<int64_t>((nout / 4 + 255) / 256, 16384)), dim3(256), 0, h->stream,
+                           (const float*)wy, Y, a);
+    else
+        hipLaunchKernelGGL((mix_unpack_kernel<float, 2>), dim3((unsigned)std::min<int64_t>((nout + 255) / 256, 16384)),
+                           dim3(256), 0, h->stream, (const float*)wy, (void*)Y, a);
     CT_HIP(hipGetLastError());
     return 0;
 }

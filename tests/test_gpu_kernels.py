@@ -288,7 +288,9 @@ def test_heff_bra_dims_differ(ctx, ora):
 
 @pytest.mark.parametrize("prec,tol", [(ct.PREC_FP32, 2e-6), (ct.PREC_BF16X3, 6e-5)])
 @pytest.mark.parametrize("shape", [(64, 64, 2, 5, 5, 5), (130, 96, 2, 8, 8, 8), (256, 256, 2, 8, 8, 8),
-                                   (64, 48, 2, 40, 40, 40), (24, 20, 2, 501, 562, 501)])
+                                   (64, 48, 2, 40, 40, 40), (20, 9, 2, 40, 40, 40), (24, 20, 2, 501, 562, 501),
+                                   # 64 pixel rows x 8 tiles of 256x256: whole batches per XCD
+                                   (64, 24, 2, 501, 562, 501)])
 def test_heff_mixed_vs_oracle(ctx, ora, prec, tol, shape):
     Dl, Dr, d, wL, wM, wR = shape
     rng = np.random.default_rng(7 + Dl)
