@@ -6,8 +6,9 @@
 // in column lane%16 (rows: see Mfma<T>::row).  The 3xBF16 path (ct_gemm_bf16.hip)
 // uses v_mfma_f32_16x16x32_bf16 with a k-contiguous LDS layout.
 //
-// Block tile 128x128, K-tile 16, 256 threads = 4 waves in a 2x2 grid, each
-// wave owning a 64x64 sub-tile = 4x4 MFMA tiles.  Operand tiles are staged in
+// Block tile 128x128 (64x64 for small problems), K-tile 16, 256 threads =
+// 4 waves in a 2x2 grid, each wave owning a 64x64 (32x32) sub-tile = 4x4 (2x2)
+// MFMA tiles.  Operand tiles are staged in
 // LDS k-major (sX[k][m]) so every fragment read is 16 consecutive elements per
 // k row.  The next K-tile is prefetched into registers while the MFMAs of the
 // current one run.  Operands may be m/n-contiguous or k-contiguous (the
@@ -60,7 +61,9 @@ struct KArgs {
     void* part;  // [batch*splits][M][N] partial sums (TC) when splits > 1
 };
 
-constexpr int BM = 128, BN = 128, BK = 16;
+// Output tiles of BT x BT (BT = 128, or 64 for problems that 128-wide tiles
+// cannot spread over the CUs), K steps of BK.
+constexpr int BK = 16;
 
 // Loads 4 consecutive elements (16-byte aligned when vec) and converts.
 template <typename TS, typename TD>
@@ -82,18 +85,20 @@ __device__ __forceinline__ void load4<float, float>(const float* p, float* o) {
 }
 
 // Tile loader.  KC == false: operand X[k][x] (x contiguous, row stride sk);
-// KC == true: operand X[x][k] (k contiguous, row stride sx).  Fills 8
-// register values per thread for a BK x 128 tile at (k0, x0).  FULL: the tile
-// is entirely inside the operand and 16-byte loads are legal (no checks).
-template <typename TS, typename TC, bool KC, bool FULL>
+// KC == true: operand X[x][k] (k contiguous, row stride sx).  Fills
+// BT*BK/256 register values per thread (8 for BT = 128, 4 for 64) for a
+// BK x BT tile at (k0, x0).  FULL: the tile is entirely inside the operand
+// and 16-byte loads are legal (no checks).
+template <typename TS, typename TC, bool KC, bool FULL, int BT>
 __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, int X0, int XN, int k0, int K,
                                           bool vec, TC* r) {
     const int tid = threadIdx.x;
     if (!KC) {
-        const int c = tid & 31, row = tid >> 5;  // cols 4c..4c+3, rows row and row+8
+        constexpr int CPR = BT / 4, RPP = 256 / CPR, PASSES = BK / RPP;  // 4-column chunks per row, rows per pass
+        const int c = tid % CPR, row = tid / CPR;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int k = k0 + row + 8 * h;
+        for (int h = 0; h < PASSES; ++h) {
+            const int k = k0 + row + RPP * h;
             const int x = X0 + 4 * c;
             TC* o = r + 4 * h;
             if (FULL || (vec && k < K && x + 3 < XN)) {
@@ -105,50 +110,55 @@ __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, i
             }
         }
     } else {
-        const int x = X0 + (tid & 127), hk = tid >> 7;  // row x, k = k0+8hk .. +7
-        const int kb = k0 + 8 * hk;
-        if (FULL || (vec && x < XN && kb + 7 < K)) {
+        constexpr int KPT = BT * BK / 256;  // k values per thread: row x, k = k0 + KPT*hk ..
+        const int x = X0 + (tid % BT), hk = tid / BT;
+        const int kb = k0 + KPT * hk;
+        if (FULL || (vec && x < XN && kb + KPT - 1 < K)) {
             const TS* p = X + (int64_t)x * sx + kb;
-            load4<TS, TC>(p, r);
-            load4<TS, TC>(p + 4, r + 4);
+#pragma unroll
+            for (int e = 0; e < KPT; e += 4) load4<TS, TC>(p + e, r + e);
         } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
+            for (int e = 0; e < KPT; ++e)
                 r[e] = (x < XN && kb + e < K) ? (TC)X[(int64_t)x * sx + (kb + e)] : (TC)0;
         }
     }
 }
 
-template <typename TC, bool KC, int LD>
+template <typename TC, bool KC, int LD, int BT>
 __device__ __forceinline__ void store_tile(TC (*S)[LD], const TC* r) {
     const int tid = threadIdx.x;
     if (!KC) {
-        const int c = tid & 31, row = tid >> 5;
+        constexpr int CPR = BT / 4, RPP = 256 / CPR, PASSES = BK / RPP;
+        const int c = tid % CPR, row = tid / CPR;
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int h = 0; h < PASSES; ++h)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) S[row + 8 * h][4 * c + e] = r[4 * h + e];
+            for (int e = 0; e < 4; ++e) S[row + RPP * h][4 * c + e] = r[4 * h + e];
     } else {
-        const int x = tid & 127, hk = tid >> 7;
+        constexpr int KPT = BT * BK / 256;
+        const int x = tid % BT, hk = tid / BT;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) S[8 * hk + e][x] = r[e];
+        for (int e = 0; e < KPT; ++e) S[KPT * hk + e][x] = r[e];
     }
 }
 
 // LDS is double-buffered: the tile for k0+BK is loaded into registers before
 // the MFMAs on k0 and stored to the other buffer after them, so each K step
 // needs a single barrier.
-template <typename TC, typename TA, typename TB, typename TO, bool AKC, bool BKC, bool FULL>
+template <typename TC, typename TA, typename TB, typename TO, bool AKC, bool BKC, bool FULL, int BT>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     if (p.active && *p.active == 0) return;
     constexpr int PAD = 4;
-    __shared__ TC sA[2][BK][BM + PAD];
-    __shared__ TC sB[2][BK][BN + PAD];
+    constexpr int NI = BT / 32;   // 16x16 accumulator blocks per wave and dimension (2 x 2 waves)
+    constexpr int NR = BT * BK / 256;
+    __shared__ TC sA[2][BK][BT + PAD];
+    __shared__ TC sB[2][BK][BT + PAD];
     typedef typename Mfma<TC>::acc_t acc_t;
 
     int tm, tn;
     gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
-    const int m0 = tm * BM, n0 = tn * BN;
+    const int m0 = tm * BT, n0 = tn * BT;
     const int zz = blockIdx.z;
     const int z = zz / p.splits;
     const int kbeg = (zz - z * p.splits) * p.kchunk;
@@ -161,43 +171,43 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     const int wm = wave >> 1, wn = wave & 1;
     const int li = lane & 15, lk = lane >> 4;
 
-    acc_t acc[4][4];
+    acc_t acc[NI][NI];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+        for (int j = 0; j < NI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    TC ra[8], rb[8];
+    TC ra[NR], rb[NR];
     const int64_t sAx = AKC ? p.sAm : 0, sAk = AKC ? 0 : p.sAk;
     const int64_t sBx = BKC ? p.sBn : 0, sBk = BKC ? 0 : p.sBk;
-    load_tile<TA, TC, AKC, FULL>(A, sAx, sAk, m0, p.M, kbeg, kend, p.vecA, ra);
-    load_tile<TB, TC, BKC, FULL>(B, sBx, sBk, n0, p.N, kbeg, kend, p.vecB, rb);
-    store_tile<TC, AKC, BM + PAD>(sA[0], ra);
-    store_tile<TC, BKC, BN + PAD>(sB[0], rb);
+    load_tile<TA, TC, AKC, FULL, BT>(A, sAx, sAk, m0, p.M, kbeg, kend, p.vecA, ra);
+    load_tile<TB, TC, BKC, FULL, BT>(B, sBx, sBk, n0, p.N, kbeg, kend, p.vecB, rb);
+    store_tile<TC, AKC, BT + PAD, BT>(sA[0], ra);
+    store_tile<TC, BKC, BT + PAD, BT>(sB[0], rb);
     __syncthreads();
 
     int cur = 0;
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
         const bool more = k0 + BK < kend;
         if (more) {
-            load_tile<TA, TC, AKC, FULL>(A, sAx, sAk, m0, p.M, k0 + BK, kend, p.vecA, ra);
-            load_tile<TB, TC, BKC, FULL>(B, sBx, sBk, n0, p.N, k0 + BK, kend, p.vecB, rb);
+            load_tile<TA, TC, AKC, FULL, BT>(A, sAx, sAk, m0, p.M, k0 + BK, kend, p.vecA, ra);
+            load_tile<TB, TC, BKC, FULL, BT>(B, sBx, sBk, n0, p.N, k0 + BK, kend, p.vecB, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
-            TC a[4], b[4];
+            TC a[NI], b[NI];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) a[i] = sA[cur][kk * 4 + lk][wm * 64 + i * 16 + li];
+            for (int i = 0; i < NI; ++i) a[i] = sA[cur][kk * 4 + lk][wm * (BT / 2) + i * 16 + li];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = sB[cur][kk * 4 + lk][wn * 64 + j * 16 + li];
+            for (int j = 0; j < NI; ++j) b[j] = sB[cur][kk * 4 + lk][wn * (BT / 2) + j * 16 + li];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < NI; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<TC>::op(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < NI; ++j) acc[i][j] = Mfma<TC>::op(a[i], b[j], acc[i][j]);
         }
         if (more) {
-            store_tile<TC, AKC, BM + PAD>(sA[cur ^ 1], ra);
-            store_tile<TC, BKC, BN + PAD>(sB[cur ^ 1], rb);
+            store_tile<TC, AKC, BT + PAD, BT>(sA[cur ^ 1], ra);
+            store_tile<TC, BKC, BT + PAD, BT>(sB[cur ^ 1], rb);
         }
         __syncthreads();
         cur ^= 1;
@@ -206,14 +216,14 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     if (p.splits > 1) {  // raw partial sums of this K range; the reduction applies alpha / beta
         TC* P = (TC*)p.part + (int64_t)zz * p.M * p.N;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < NI; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int n = n0 + wn * 64 + j * 16 + li;
+            for (int j = 0; j < NI; ++j) {
+                const int n = n0 + wn * (BT / 2) + j * 16 + li;
                 if (!FULL && n >= p.N) continue;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int m = m0 + wm * 64 + i * 16 + Mfma<TC>::row(lk, r);
+                    const int m = m0 + wm * (BT / 2) + i * 16 + Mfma<TC>::row(lk, r);
                     if (FULL || m < p.M) P[(int64_t)m * p.N + n] = acc[i][j][r];
                 }
             }
@@ -221,14 +231,14 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     }
     const double alpha = p.alpha, beta = p.beta;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < NI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int n = n0 + wn * 64 + j * 16 + li;
+        for (int j = 0; j < NI; ++j) {
+            const int n = n0 + wn * (BT / 2) + j * 16 + li;
             if (!FULL && n >= p.N) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int m = m0 + wm * 64 + i * 16 + Mfma<TC>::row(lk, r);
+                const int m = m0 + wm * (BT / 2) + i * 16 + Mfma<TC>::row(lk, r);
                 if (FULL || m < p.M) {
                     TO* c = C + (int64_t)m * p.ldc + n;
                     double v = alpha * (double)acc[i][j][r];
@@ -264,13 +274,13 @@ bool aligned16(const void* ptr, int64_t s1, int64_t s2) {
     return ((uintptr_t)ptr % 16 == 0) && (s1 % q == 0) && (s2 % q == 0);
 }
 
-template <typename TC, typename TA, typename TB, typename TO>
-int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDesc& g) {
+template <typename TC, typename TA, typename TB, typename TO, int BT>
+int launch_bt(Handle* h, const void* A, const void* B, void* C, const GemmDesc& g) {
     KArgs p;
     p.A = A; p.B = B; p.C = C;
     p.M = g.M; p.N = g.N; p.K = g.K;
-    p.tilesM = (g.M + BM - 1) / BM;
-    p.tilesN = (g.N + BN - 1) / BN;
+    p.tilesM = (g.M + BT - 1) / BT;
+    p.tilesN = (g.N + BT - 1) / BT;
     p.sAm = g.sAm; p.sAk = g.sAk; p.sBk = g.sBk; p.sBn = g.sBn;
     p.ldc = g.ldc; p.bsA = g.bsA; p.bsB = g.bsB; p.bsC = g.bsC;
     p.alpha = g.alpha; p.beta = g.beta; p.active = g.active;
@@ -292,9 +302,9 @@ int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDes
     p.part = nullptr;
     if (splits > 1) CT_CHECK(ws_get(h, WS_SPLITK, (size_t)g.batch * splits * g.M * g.N * sizeof(TC), &p.part));
     dim3 grid(p.tilesM * p.tilesN, 1, g.batch * splits), block(256);
-    const bool full = p.vecA && p.vecB && (g.M % BM == 0) && (g.N % BN == 0) && (g.K % BK == 0);
+    const bool full = p.vecA && p.vecB && (g.M % BT == 0) && (g.N % BT == 0) && (g.K % BK == 0);
 #define CT_LAUNCH(AK, BKc, F) \
-    hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, AK, BKc, F>), grid, block, 0, h->stream, p)
+    hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, AK, BKc, F, BT>), grid, block, 0, h->stream, p)
     if (full) {
         if (!akc && !bkc) CT_LAUNCH(false, false, true);
         else if (akc && !bkc) CT_LAUNCH(true, false, true);
@@ -316,6 +326,17 @@ int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDes
         CT_HIP(hipGetLastError());
     }
     return 0;
+}
+
+// 64x64 tiles when 128x128 tiles would occupy fewer than half the CUs and
+// split-K cannot make up for it (K < 2048, or fewer than 16 tiles): four times
+// the workgroups for the same problem.  With long K the 128x128 tiles split
+// over K fill the GPU at a better MFMA-to-load ratio (measured at D = 512).
+template <typename TC, typename TA, typename TB, typename TO>
+int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDesc& g) {
+    const int64_t tiles128 = (int64_t)((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch;
+    if (tiles128 < 128 && (g.K < 2048 || tiles128 < 16)) return launch_bt<TC, TA, TB, TO, 64>(h, A, B, C, g);
+    return launch_bt<TC, TA, TB, TO, 128>(h, A, B, C, g);
 }
 
 template <typename TC>
