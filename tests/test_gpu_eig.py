@@ -66,7 +66,8 @@ def run_sytrd(ctx, a):
     return A.cpu().numpy(), d.cpu().numpy(), e.cpu().numpy()[:n - 1], tau.cpu().numpy()[:n - 1]
 
 
-@pytest.mark.parametrize("n,kind", [(2, "sym"), (3, "sym"), (17, "sym"), (64, "gram"), (255, "sym"), (256, "sym"),
+@pytest.mark.parametrize("n,kind", [(2, "sym"), (3, "sym"), (17, "sym"), (64, "gram"), (184, "sym"), (185, "sym"),
+                                    (255, "sym"), (256, "sym"),
                                     (257, "gram"), (500, "degenerate"), (1000, "sym"), (1024, "gram"),
                                     (2048, "sym")])
 def test_sytrd_vs_lapack(ctx, n, kind):
@@ -105,7 +106,8 @@ def test_sytrd_vs_lapack(ctx, n, kind):
     assert np.abs(lam - np.linalg.eigvalsh(a)).max() <= tol
 
 
-@pytest.mark.parametrize("n,kind,method", [(1, "sym", 0), (2, "gram", 0), (100, "degenerate", 0), (513, "sym", 0),
+@pytest.mark.parametrize("n,kind,method", [(1, "sym", 0), (2, "gram", 0), (100, "degenerate", 0), (164, "gram", 0),
+                                           (513, "sym", 0),
                                            (1024, "degenerate", 0), (2048, "gram", 0), (2100, "sym", 0),
                                            (300, "gram", 1)])
 def test_syevd_vs_numpy(ctx, n, kind, method):
