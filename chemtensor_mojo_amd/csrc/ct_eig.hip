@@ -579,23 +579,23 @@ int syevd_stage2(Handle* h, int64_t n, double* Z, int64_t rows) {
     const double* M = (const double*)wm;
     double* W = (double*)ww;
     double* E = Z + (n - rows) * n;
-    const double one = 1.0, zero = 0.0, minus_one = -1.0;
     for (int m = nb - 1; m >= 0; --m) {
-        const int64_t off = (int64_t)m * BT_B + 1;  // Vt_m is zero in columns < off
+        // Vt_m and M_m are zero in columns < m*B + 1; starting at m*B keeps the
+        // operand rows 16-byte aligned and adds only zero products
+        const int64_t off = (int64_t)m * BT_B;
         const int64_t K = n - off;
-        // W^T (B x rows) = Vt_m[:, off:] E[:, off:]^T   (column-major views)
-        rocblas_status st = rocblas_dgemm(h->blas, rocblas_operation_transpose, rocblas_operation_none, BT_B,
-                                          (rocblas_int)rows, (rocblas_int)K, &one, Vt + (int64_t)m * BT_B * n + off,
-                                          (rocblas_int)n, E + off, (rocblas_int)n, &zero, W, BT_B);
-        // E[:, off:] -= W M_m[:, off:]
-        if (st == rocblas_status_success)
-            st = rocblas_dgemm(h->blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)K,
-                               (rocblas_int)rows, BT_B, &minus_one, M + (int64_t)m * BT_B * n + off, (rocblas_int)n, W,
-                               BT_B, &one, E + off, (rocblas_int)n);
-        if (st != rocblas_status_success) {
-            set_last_error(std::string("rocblas_dgemm (back-transformation): ") + rocblas_status_to_string(st));
-            return CT_STATUS_ROCBLAS_ERROR_BASE + (int)st;
-        }
+        GemmDesc g1;  // W[rows][B] = E[:, off:] Vt_m[:, off:]^T (E: the kept rows of Z)
+        g1.M = (int)rows; g1.N = BT_B; g1.K = (int)K;
+        g1.a_kc = 1; g1.sAm = n; g1.sAk = 1;
+        g1.b_kc = 1; g1.sBn = n; g1.sBk = 1;
+        g1.ldc = BT_B;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, E + off, CT_F64, Vt + (int64_t)m * BT_B * n + off, CT_F64, W, g1));
+        GemmDesc g2;  // E[:, off:] -= W M_m[:, off:]
+        g2.M = (int)rows; g2.N = (int)K; g2.K = BT_B;
+        g2.a_kc = 1; g2.sAm = BT_B; g2.sAk = 1;
+        g2.b_kc = 0; g2.sBk = n; g2.sBn = 1;
+        g2.ldc = n; g2.alpha = -1.0; g2.beta = 1.0;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, W, CT_F64, M + (int64_t)m * BT_B * n + off, CT_F64, E + off, g2));
     }
     return 0;
 }
