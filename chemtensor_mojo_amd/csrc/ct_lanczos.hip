@@ -10,8 +10,9 @@
 //
 // Mixed-precision matvecs (no reference counterpart) carry a relative error
 // far above DBL_EPSILON, so for them the breakdown test also fires once
-// beta_j falls below that error level times max_i |alpha_i| (1e-6 for FP32,
-// 1e-5 for 3xBF16): past that point new Krylov directions are matvec noise.
+// beta_j falls below 1e-5 times max_i |alpha_i| (FP32 and 3xBF16; their
+// matvec errors reach 5e-6 and 8e-6 at the largest MPO bonds): past that
+// point new Krylov directions are matvec noise.
 // The FP64 path keeps the reference test unchanged.
 //
 // Optional refinement for mixed-precision matvecs (no reference counterpart):
@@ -514,7 +515,7 @@ int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta
     LzState* s = (LzState*)sb;
     const int* act = &s->active;
     const double thr = 100.0 * (double)dim * 2.2204460492503131e-16;
-    const double floor_rel = env.prec == CT_PREC_FP32 ? 1e-6 : (env.prec == CT_PREC_BF16X3 ? 1e-5 : 0.0);
+    const double floor_rel = env.prec == CT_PREC_FP64 ? 0.0 : 1e-5;
     hipStream_t st = h->stream;
     const int G = grid_n(dim);
 
