@@ -160,8 +160,8 @@ int ct_two_site_theta(ct_handle_t h, int prec, int dtype, const void* A_i, const
  * normalised FP64 Ritz vector).  theta (FP64, [Dl,d,d,Dr]) is the start
  * vector on entry and the Ritz vector on return.
  * L, R, Wi, Wj are FP64.  prec selects the matvec precision; the FP32 and
- * 3xBF16 forms also stop once beta < 1e-6 (FP32) or 1e-5 (3xBF16) times
- * max |alpha|, the noise level of their matvecs.  refine != 0 adds
+ * 3xBF16 forms also stop once beta < 1e-5 times max |alpha|, the noise
+ * level of their matvecs.  refine != 0 adds
  * an FP64 Rayleigh quotient of the Ritz vector (E_rq) and, refine == 2, one
  * residual-correction step in span{psi, H psi - E_rq psi}; E0 then returns the
  * refined energy.  Synchronises the stream.  num_iter = Krylov dimension. */
