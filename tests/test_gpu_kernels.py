@@ -598,3 +598,23 @@ def test_dmrg_quantum_chemistry_vs_fci(ctx, ora, prec, refine, tol):
     if prec == ct.PREC_FP64:
         ref = ora.dmrg_two_site(mps_t, mpo_t, 6, 16, 0.0, 20, 1e-12, 0.0, jacobi_max_rot=20000)
         assert abs(E - ref["E"]) <= 1e-10 * abs(ref["E"])
+
+
+@pytest.mark.parametrize("prec,refine", [(ct.PREC_FP64, 0), (ct.PREC_BF16X3, 2)])
+def test_dmrg_quantum_chemistry_number_sector_start(ctx, prec, refine):
+    """Start inside the 4-electron sector (random MPS with particle-number
+    block structure): the sweeps reach full CI and the state stays in the
+    sector, <(N - 4)^2> ~ 0, measured with the MPO of the penalty alone."""
+    mpo_t, info = ham.qc_tensors(4, seed=21, penalty=1.0)
+    pen_t = ham.qc_mpo_tensors(np.zeros((4, 4)), np.zeros((4,) * 4), 1.0, 4)
+    start = ham.number_sector_random_mps_tensors(8, 4, 16, seed=8)
+    p = ct.DMRGParams(num_sweeps=6, chi_max=16, eps_trunc=0.0, max_krylov_iter=20, krylov_tol=1e-12,
+                      energy_tol=0.0, jacobi_max_rot=-1, prec=prec, refine=refine)
+    E, out = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, start), p)
+    assert abs(E - info["E_fci"]) <= 1e-9 * abs(info["E_fci"]), (E, info["E_fci"])
+    psi = out.to_numpy()[0]
+    for A in out.to_numpy()[1:]:
+        psi = np.tensordot(psi, A, axes=([-1], [0]))
+    psi = psi.reshape(-1)
+    leak = float(psi @ (ham.mpo_to_dense(pen_t) @ psi) / (psi @ psi))
+    assert leak < 1e-10, leak
