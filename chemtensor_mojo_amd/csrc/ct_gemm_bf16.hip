@@ -20,10 +20,11 @@
 //    in AGPRs (one wave per SIMD).  Operand tiles travel HBM/L2 -> LDS by
 //    direct-to-LDS loads (global_load_lds_dwordx4) into two 64 KiB stages: the
 //    next stage is in flight while the MFMAs read the current one, with no
-//    VGPR staging and no LDS write instructions.  Inside a stage the A
-//    fragments of row block i+1 are read while the MFMAs of block i run.
-//    Optional split-K over blockIdx.z with FP32 partial sums reduced by a
-//    second kernel.
+//    VGPR staging and no LDS write instructions.  In each K step all
+//    fragment reads are issued between the first MFMAs, before the copy of
+//    the next stage (see the main loop).  Optional split-K over blockIdx.z
+//    with FP32 partial sums reduced by a second kernel; split-K launches give
+//    each XCD one K range and a compact block of tiles.
 //  * 128x128x32 blocks, 256 threads = 4 waves (2x2) of 64x64, LDS
 //    double-buffered through registers, for problems too small to give every
 //    CU a 256x256 tile.
@@ -157,7 +158,7 @@ struct NtArgs {
     const int* active;
     int GM;                 // tile rows per L2 group (see gemm_tile_coords)
     int splits;             // split-K factor
-    int batch_xcd;          // 256x256 kernel: whole batches per XCD (gemm_batch_per_xcd)
+    int batch_xcd;          // 256x256 kernel: 1 = whole batches per XCD (gemm_batch_per_xcd), 2 = one K range per XCD
     int kchunk;             // K range per split, multiple of 32
     float* part;            // [batch*splits][M][N] FP32 partial sums when splits > 1
 };
@@ -186,7 +187,20 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_dma_kernel(NtArgs p) {
     __shared__ __attribute__((aligned(1024))) char smem[2 * STAGE];
 
     int tm, tn, b, sk, z;
-    if (p.batch_xcd) {  // splits == 1
+    if (p.batch_xcd == 2) {
+        // Split-K launches (grid = tiles x 1 x batch*splits, a multiple of 8
+        // workgroups): workgroup f runs on XCD f mod 8, so give each XCD a
+        // contiguous run of (z, tile) pairs.  The workgroups of one XCD then
+        // share one K range and a compact block of tiles, and their A and B
+        // panels are fetched into that XCD's L2 once.  A bijection.
+        const int nt = gridDim.x, total = nt * gridDim.z;
+        const int f = blockIdx.z * nt + blockIdx.x;
+        const int v = (f & 7) * (total >> 3) + (f >> 3);
+        z = v / nt;
+        gemm_group_coords(v - z * nt, p.tilesM, p.tilesN, p.GM, tm, tn);
+        b = z / p.splits;
+        sk = z - b * p.splits;
+    } else if (p.batch_xcd) {  // splits == 1
         int tile;
         gemm_batch_per_xcd(gridDim.x, b, tile);
         gemm_group_coords(tile, p.tilesM, p.tilesN, p.GM, tm, tn);
@@ -237,12 +251,18 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_dma_kernel(NtArgs p) {
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
+    // One K step: every LDS fragment read of the step is issued first (the lo
+    // B fragments between the first eight MFMAs, the A fragments of row
+    // blocks 1-7 between the next sixteen), and only then the direct-to-LDS
+    // copy of the next stage, between the MFMAs of row block 1.  While such a
+    // copy is in flight the compiler can only wait for LDS reads with
+    // lgkmcnt(0), so this order is what lets the fragment reads overlap the
+    // MFMAs.  Every accumulator still sees l*bh, h*bl, h*bh in that order.
     if (kbeg < kend) issue(0, kbeg);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     int stage = 0;
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        if (k0 + BK < kend) issue(stage ^ 1, k0 + BK);
         const char* sa = smem + stage * STAGE;
         const char* sb = sa + SOPND;
         auto rdA = [&](int i, v8bf& h, v8bf& l) {
@@ -250,27 +270,60 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_dma_kernel(NtArgs p) {
             h = __builtin_bit_cast(v8bf, *(const v4u*)(sa + r * SROW + ((lk ^ (r & 7)) * 16)));
             l = __builtin_bit_cast(v8bf, *(const v4u*)(sa + r * SROW + (((lk + 4) ^ (r & 7)) * 16)));
         };
-        v8bf ah[2], al[2];
-        rdA(0, ah[0], al[0]);
-        v8bf bh[8], bl[8];
+        auto rdB = [&](int j, int hl) -> v8bf {
+            const int r = wn * 128 + j * 16 + li;
+            return __builtin_bit_cast(v8bf, *(const v4u*)(sb + r * SROW + (((lk + 4 * hl) ^ (r & 7)) * 16)));
+        };
+        // the last step copies its own block again (in bounds, never read)
+        const int knext = min(k0 + BK, kend - BK);
+        v8bf aH[8], aL[8], bh[8], bl[8];
+        rdA(0, aH[0], aL[0]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bh[j] = rdB(j, 0);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            const int r = wn * 128 + j * 16 + li;
-            bh[j] = __builtin_bit_cast(v8bf, *(const v4u*)(sb + r * SROW + ((lk ^ (r & 7)) * 16)));
-            bl[j] = __builtin_bit_cast(v8bf, *(const v4u*)(sb + r * SROW + (((lk + 4) ^ (r & 7)) * 16)));
+            acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aL[0], bh[j], acc[0][j], 0, 0, 0);
+            bl[j] = rdB(j, 1);
         }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            if (i + 1 < 8) rdA(i + 1, ah[(i + 1) & 1], al[(i + 1) & 1]);
-            // keep the reads of block i+1 ahead of the MFMAs of block i
-            __builtin_amdgcn_sched_barrier(0);
-            const v8bf h = ah[i & 1], l = al[i & 1];
+        for (int j = 0; j < 8; ++j) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 1; i < 8; ++i) rdA(i, aH[i], aL[i]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH[0], bl[j], acc[0][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH[0], bh[j], acc[0][j], 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 14; ++q) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 1; i < 8; ++i) {
+            if (i == 1) issue(stage ^ 1, knext);
+            const v8bf h = aH[i], l = aL[i];
 #pragma unroll
             for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l, bh[j], acc[i][j], 0, 0, 0);
 #pragma unroll
             for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h, bl[j], acc[i][j], 0, 0, 0);
 #pragma unroll
             for (int j = 0; j < 8; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h, bh[j], acc[i][j], 0, 0, 0);
+            if (i == 1) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one global read
+                }
+                __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
+            }
             __builtin_amdgcn_sched_barrier(0);
         }
         // the next stage has landed in LDS (this wave's copies), then every wave's
@@ -323,6 +376,60 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restr
         double v = alpha * (double)s;
         if (beta != 0.0) v += beta * (double)(*c);
         *c = (TO)v;
+    }
+}
+
+// The same reduction, four consecutive n per thread (16-byte partial loads,
+// 16-byte stores): needs N % 4 == 0 and C rows 16-byte aligned (the host
+// checks).  Each element sums its partials in the same order as above.
+template <typename TO>
+__global__ __launch_bounds__(256) void splitk_reduce4_kernel(const float* __restrict__ part, TO* __restrict__ C, int M,
+                                                             int N, int splits, int batch, int64_t ldc, int64_t bsC,
+                                                             double alpha, double beta, const int* active) {
+    if (active && *active == 0) return;
+    const int64_t MN = (int64_t)M * N;
+    const int64_t total4 = MN * batch / 4;
+    for (int64_t e4 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e4 < total4;
+         e4 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = 4 * e4;
+        const int64_t b = e / MN, r = e - b * MN;
+        const int64_t m = r / N, n = r - m * N;
+        const float* q = part + b * splits * MN + r;
+        v4f s = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int k = 0; k < splits; ++k) s += *(const v4f*)(q + k * MN);
+        TO* c = C + b * bsC + m * ldc + n;
+        double v[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) v[t] = alpha * (double)s[t];
+        if (beta != 0.0) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] += beta * (double)c[t];
+        }
+        if constexpr (sizeof(TO) == 8) {
+            typedef double v2d __attribute__((ext_vector_type(2)));
+            *(v2d*)c = v2d{v[0], v[1]};
+            *(v2d*)(c + 2) = v2d{v[2], v[3]};
+        } else {
+            *(v4f*)c = v4f{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+        }
+    }
+}
+
+// C = alpha * sum of the split partials + beta * C, vectorized when the
+// shapes and alignment allow.
+template <typename TO>
+void splitk_reduce(hipStream_t st, const float* part, TO* C, int M, int N, int splits, int batch, int64_t ldc,
+                   int64_t bsC, double alpha, double beta, const int* active) {
+    const int64_t total = (int64_t)M * N * batch;
+    const int64_t vecw = 16 / (int64_t)sizeof(TO);  // elements per 16 bytes
+    if (N % 4 == 0 && ldc % vecw == 0 && bsC % vecw == 0 && (uintptr_t)C % 16 == 0) {
+        const int g = (int)std::min<int64_t>((total / 4 + 255) / 256, 8192);
+        hipLaunchKernelGGL(splitk_reduce4_kernel<TO>, dim3(g), dim3(256), 0, st, part, C, M, N, splits, batch, ldc,
+                           bsC, alpha, beta, active);
+    } else {
+        const int g = (int)std::min<int64_t>((total + 255) / 256, 8192);
+        hipLaunchKernelGGL(splitk_reduce_kernel<TO>, dim3(g), dim3(256), 0, st, part, C, M, N, splits, batch, ldc, bsC,
+                           alpha, beta, active);
     }
 }
 
@@ -535,6 +642,14 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
         if (p.GM < 1) p.GM = 1;
         p.splits = splits;
         p.kchunk = (K32 / splits + BK - 1) / BK * BK;
+        const int64_t total = (int64_t)p.tilesM * p.tilesN * batch * splits;
+        if (splits > 1 && total % 8 == 0) {
+            // one K range per XCD (gemm coords mode 2): the XCD's share of one
+            // z-plane, grouped into whole rows of tiles
+            const int64_t share = std::min<int64_t>((int64_t)p.tilesM * p.tilesN, total / 8);
+            p.batch_xcd = 2;
+            p.GM = share >= p.tilesN ? (int)(share / p.tilesN) : 1;
+        }
         // few tiles per batch: keep each batch on one XCD so its panels are fetched once
         if (splits == 1 && batch % 8 == 0 && p.tilesM * p.tilesN <= 32) {
             p.batch_xcd = 1;
@@ -546,16 +661,12 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
             CT_CHECK(ws_get(h, WS_SPLITK, (size_t)batch * splits * M * N * 4, &w));
             p.part = (float*)w;
             hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<float, true>), grid, block, 0, h->stream, p);
-            const int64_t total = (int64_t)M * N * batch;
-            const int g = (int)std::min<int64_t>((total + 255) / 256, 8192);
             if (tC == CT_F64)
-                hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(g), dim3(256), 0, h->stream,
-                                   (const float*)p.part, (double*)C, M, N, splits, batch, ldc, bsC, alpha, beta,
-                                   active);
+                splitk_reduce<double>(h->stream, p.part, (double*)C, M, N, splits, batch, ldc, bsC, alpha, beta,
+                                      active);
             else
-                hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(g), dim3(256), 0, h->stream,
-                                   (const float*)p.part, (float*)C, M, N, splits, batch, ldc, bsC, alpha, beta,
-                                   active);
+                splitk_reduce<float>(h->stream, p.part, (float*)C, M, N, splits, batch, ldc, bsC, alpha, beta,
+                                     active);
         } else if (tC == CT_F64) {
             hipLaunchKernelGGL((gemm_bf16x3_dma_kernel<double, false>), grid, block, 0, h->stream, p);
         } else {
@@ -581,14 +692,10 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
         CT_CHECK(ws_get(h, WS_SPLITK, (size_t)batch * s128 * M * N * 4, &w));
         p.part = (float*)w;
         hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<float>, grid, block, 0, h->stream, p);
-        const int64_t total = (int64_t)M * N * batch;
-        const int g = (int)std::min<int64_t>((total + 255) / 256, 8192);
         if (tC == CT_F64)
-            hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(g), dim3(256), 0, h->stream, (const float*)p.part,
-                               (double*)C, M, N, s128, batch, ldc, bsC, alpha, beta, active);
+            splitk_reduce<double>(h->stream, p.part, (double*)C, M, N, s128, batch, ldc, bsC, alpha, beta, active);
         else
-            hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(g), dim3(256), 0, h->stream, (const float*)p.part,
-                               (float*)C, M, N, s128, batch, ldc, bsC, alpha, beta, active);
+            splitk_reduce<float>(h->stream, p.part, (float*)C, M, N, s128, batch, ldc, bsC, alpha, beta, active);
     } else if (tC == CT_F64) {
         hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<double>, grid, block, 0, h->stream, p);
     } else {
