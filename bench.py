@@ -35,10 +35,10 @@ sys.path.insert(0, ROOT)
 METRIC = "DMRG sweep s + H_eff matvec TFLOP/s at D=1024; |ΔE/E| vs FP64 oracle"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # TFLOP/s, MI355X dense MFMA
 HBM_PEAK = 8000.0  # GB/s
-# HBM bytes of one call of the dominant GEMM at D=1024, w=8 (GEMM: 404.6 MB read +
-# 67.2 MB written; split-K reduction: 67.1 MB + 33.6 MB), from the FETCH_SIZE and
+# HBM bytes of one call of the dominant GEMM at D=1024, w=8 (GEMM: 294.9 MB read +
+# 67.3 MB written; split-K reduction: 67.1 MB + 33.6 MB), from the FETCH_SIZE and
 # WRITE_SIZE counter runs in profiles/heff_matvec_j1j2_D1024_hbm_bytes.md.
-DOMINANT_TRAFFIC_BYTES = 572.5e6
+DOMINANT_TRAFFIC_BYTES = 462.9e6
 
 
 def log(*a):
@@ -716,7 +716,7 @@ def main():
         "matvec": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in mv.items()},
         "roofline": {"bound": "mfma",
-                     "kernel": "gemm_bf16x3_dma_kernel<float,true> + splitk_reduce_kernel<double> "
+                     "kernel": "gemm_bf16x3_dma_kernel<float,true> + splitk_reduce4_kernel<double> "
                                f"(last GEMM of the matvec, [{4 * D}x{w * D}].[{w * D}x{D}], split-K 4)",
                      "achieved": round(dom["tflops"], 3), "peak": PEAK["bf16"] / 3.0, "unit": "TFLOP/s",
                      "frac": round(dom["tflops"] / (PEAK["bf16"] / 3.0), 4),
