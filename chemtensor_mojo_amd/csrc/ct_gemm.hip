@@ -84,11 +84,25 @@ __device__ __forceinline__ void load4<float, float>(const float* p, float* o) {
     o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
 }
 
+// Loads the 16 / sizeof(TS) consecutive elements of one 16-byte vector and converts.
+template <typename TS, typename TD>
+__device__ __forceinline__ void load16(const TS* p, TD* o) {
+    if constexpr (sizeof(TS) == 8) {
+        v2d a = *(const v2d*)p;
+        o[0] = (TD)a.x; o[1] = (TD)a.y;
+    } else {
+        v4f a = *(const v4f*)p;
+        o[0] = (TD)a.x; o[1] = (TD)a.y; o[2] = (TD)a.z; o[3] = (TD)a.w;
+    }
+}
+
 // Tile loader.  KC == false: operand X[k][x] (x contiguous, row stride sk);
 // KC == true: operand X[x][k] (k contiguous, row stride sx).  Fills
 // BT*BK/256 register values per thread (8 for BT = 128, 4 for 64) for a
 // BK x BT tile at (k0, x0).  FULL: the tile is entirely inside the operand
-// and 16-byte loads are legal (no checks).
+// and 16-byte loads are legal (no checks).  For KC, the BK elements of one
+// row are read by BK / VE adjacent lanes (VE elements = 16 bytes each), so
+// one load instruction covers whole rows' K slices.
 template <typename TS, typename TC, bool KC, bool FULL, int BT>
 __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, int X0, int XN, int k0, int K,
                                           bool vec, TC* r) {
@@ -110,22 +124,24 @@ __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, i
             }
         }
     } else {
-        constexpr int KPT = BT * BK / 256;  // k values per thread: row x, k = k0 + KPT*hk ..
-        const int x = X0 + (tid % BT), hk = tid / BT;
-        const int kb = k0 + KPT * hk;
-        if (FULL || (vec && x < XN && kb + KPT - 1 < K)) {
-            const TS* p = X + (int64_t)x * sx + kb;
+        constexpr int VE = 16 / (int)sizeof(TS), LPR = BK / VE, RPP = 256 / LPR, PASSES = BT / RPP;
+        const int kq = tid % LPR, xr = tid / LPR;
+        const int k = k0 + VE * kq;
 #pragma unroll
-            for (int e = 0; e < KPT; e += 4) load4<TS, TC>(p + e, r + e);
-        } else {
+        for (int h = 0; h < PASSES; ++h) {
+            const int x = X0 + xr + RPP * h;
+            TC* o = r + VE * h;
+            if (FULL || (vec && x < XN && k + VE - 1 < K)) {
+                load16<TS, TC>(X + (int64_t)x * sx + k, o);
+            } else {
 #pragma unroll
-            for (int e = 0; e < KPT; ++e)
-                r[e] = (x < XN && kb + e < K) ? (TC)X[(int64_t)x * sx + (kb + e)] : (TC)0;
+                for (int e = 0; e < VE; ++e) o[e] = (x < XN && k + e < K) ? (TC)X[(int64_t)x * sx + (k + e)] : (TC)0;
+            }
         }
     }
 }
 
-template <typename TC, bool KC, int LD, int BT>
+template <typename TS, typename TC, bool KC, int LD, int BT>
 __device__ __forceinline__ void store_tile(TC (*S)[LD], const TC* r) {
     const int tid = threadIdx.x;
     if (!KC) {
@@ -136,10 +152,12 @@ __device__ __forceinline__ void store_tile(TC (*S)[LD], const TC* r) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) S[row + RPP * h][4 * c + e] = r[4 * h + e];
     } else {
-        constexpr int KPT = BT * BK / 256;
-        const int x = tid % BT, hk = tid / BT;
+        constexpr int VE = 16 / (int)sizeof(TS), LPR = BK / VE, RPP = 256 / LPR, PASSES = BT / RPP;
+        const int kq = tid % LPR, xr = tid / LPR;
 #pragma unroll
-        for (int e = 0; e < KPT; ++e) S[KPT * hk + e][x] = r[e];
+        for (int h = 0; h < PASSES; ++h)
+#pragma unroll
+            for (int e = 0; e < VE; ++e) S[VE * kq + e][xr + RPP * h] = r[VE * h + e];
     }
 }
 
@@ -182,8 +200,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     const int64_t sBx = BKC ? p.sBn : 0, sBk = BKC ? 0 : p.sBk;
     load_tile<TA, TC, AKC, FULL, BT>(A, sAx, sAk, m0, p.M, kbeg, kend, p.vecA, ra);
     load_tile<TB, TC, BKC, FULL, BT>(B, sBx, sBk, n0, p.N, kbeg, kend, p.vecB, rb);
-    store_tile<TC, AKC, BT + PAD, BT>(sA[0], ra);
-    store_tile<TC, BKC, BT + PAD, BT>(sB[0], rb);
+    store_tile<TA, TC, AKC, BT + PAD, BT>(sA[0], ra);
+    store_tile<TB, TC, BKC, BT + PAD, BT>(sB[0], rb);
     __syncthreads();
 
     int cur = 0;
@@ -206,8 +224,8 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
                 for (int j = 0; j < NI; ++j) acc[i][j] = Mfma<TC>::op(a[i], b[j], acc[i][j]);
         }
         if (more) {
-            store_tile<TC, AKC, BT + PAD, BT>(sA[cur ^ 1], ra);
-            store_tile<TC, BKC, BT + PAD, BT>(sB[cur ^ 1], rb);
+            store_tile<TA, TC, AKC, BT + PAD, BT>(sA[cur ^ 1], ra);
+            store_tile<TB, TC, BKC, BT + PAD, BT>(sB[cur ^ 1], rb);
         }
         __syncthreads();
         cur ^= 1;
@@ -290,9 +308,14 @@ int launch_bt(Handle* h, const void* A, const void* B, void* C, const GemmDesc& 
     p.vecB = bkc ? aligned16<TB>(B, g.sBn, g.bsB) : aligned16<TB>(B, g.sBk, g.bsB);
     // Split K when the tiles alone would leave most CUs idle (small bond
     // dimensions): at least 8 K-tiles per split, aiming at >= 256 workgroups.
+    // With long K, also when they give fewer than two workgroups per CU: a
+    // lone workgroup has nothing to run while it waits at its per-step
+    // barrier (the [4096 x 8192] . [8192 x 1024] product of the D = 1024
+    // matvec is 256 tiles of 128 x 128).
     const int64_t tiles = (int64_t)p.tilesM * p.tilesN * g.batch;
     int splits = 1;
     if (tiles < 128 && g.K >= 16 * BK) splits = (int)std::min<int64_t>((256 + tiles - 1) / tiles, g.K / (8 * BK));
+    else if (tiles < 512 && g.K >= 128 * BK) splits = (int)std::min<int64_t>((512 + tiles - 1) / tiles, g.K / (64 * BK));
     p.kchunk = g.K;
     if (splits > 1) {
         p.kchunk = (int)(((g.K + splits - 1) / splits + BK - 1) / BK * BK);
