@@ -45,6 +45,10 @@ DOT_CASES = [
     ((64, 4096), (4096, 96), 1, False, True),
     ((3, 5000), (7, 5000), 1, False, False),
     ((2, 3, 1000), (3, 1000, 130), 2, False, True),
+    # 128-192 output tiles of 128x128 and K >= 2048: split K for two workgroups per CU;
+    # k-contiguous A (and B), ragged K through the row-slice tile loader
+    ((1024, 4096), (4096, 2048), 1, False, True),
+    ((2048, 3000), (1536, 3000), 1, False, False),
 ]
 
 
