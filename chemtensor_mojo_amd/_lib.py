@@ -29,14 +29,14 @@ class DMRGParamsC(ctypes.Structure):
     _fields_ = [("num_sweeps", _int), ("chi_max", _i64), ("eps_trunc", _dbl), ("max_krylov_iter", _int),
                 ("krylov_tol", _dbl), ("energy_tol", _dbl), ("jacobi_max_rot", _int), ("svd_mode", _int),
                 ("prec", _int), ("refine", _int),
-                ("env_prec", _int), ("verbose", _int)]
+                ("env_prec", _int), ("verbose", _int), ("graphs", _int)]
 
 
 class DMRGStatsC(ctypes.Structure):
     _fields_ = [("energy", _dbl), ("sweeps", _int), ("local_updates", _int), ("trunc_weight_sum", _dbl),
                 ("trunc_weight_max", _dbl), ("max_bond", _i64), ("t_total", _dbl), ("t_lanczos", _dbl),
                 ("t_svd", _dbl), ("t_env", _dbl), ("t_theta", _dbl), ("matvecs", _i64),
-                ("matvec_flops", _dbl), ("env_flops", _dbl)]
+                ("matvec_flops", _dbl), ("env_flops", _dbl), ("graph_replays", _i64)]
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)

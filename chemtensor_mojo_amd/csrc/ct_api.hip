@@ -13,8 +13,16 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 int buf_get(Handle* h, Buffer& b, size_t bytes, void** out) {
     if (bytes == 0) bytes = 16;
     if (b.bytes < bytes) {
+        // never inside a stream capture: the synchronisation and the new
+        // address would break the graph being recorded
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (h->stream && hipStreamIsCapturing(h->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+            set_last_error("workspace growth during graph capture");
+            return CT_STATUS_NOT_SUPPORTED;
+        }
         // earlier work on the stream may still read the old buffer
         CT_HIP(hipStreamSynchronize(h->stream));
+        ++h->ws_gen;
         if (b.ptr) CT_HIP(hipFree(b.ptr));
         b.ptr = nullptr;
         b.bytes = 0;

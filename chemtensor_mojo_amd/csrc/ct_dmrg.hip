@@ -12,8 +12,10 @@
 // min(d^l, d^(N-l), chi_max) (or the initial bond when larger), so a sweep
 // performs no allocation.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 
 #include "ct_internal.h"
 
@@ -24,6 +26,19 @@ double now_s() {
 }
 
 }  // namespace
+
+// The Lanczos solve of one bond as a HIP graph.  key: everything the
+// recorded kernel arguments depend on (shapes, precision, solver settings and
+// the bond's device buffers); gen: the handle's workspace generation at
+// capture (a reallocated workspace makes the recorded addresses stale).
+struct BondGraph {
+    std::array<int64_t, 16> seen_key{};  // key of the last eager run: every workspace is large enough for it
+    bool seen = false;
+    bool no_capture = false;             // capture failed once: stay eager for this bond
+    std::array<int64_t, 16> exec_key{};  // key the graph was captured for
+    uint64_t gen = 0;
+    hipGraphExec_t exec = nullptr;
+};
 
 struct ct_dmrg_s {
     ct::Handle* h = nullptr;
@@ -36,9 +51,15 @@ struct ct_dmrg_s {
     double* S = nullptr;
     ct_dmrg_stats st{};
     std::vector<double> sweep_E;
+    std::vector<BondGraph> graphs;
+    hipStream_t gstream = nullptr;  // capture stream (the caller's stream may be the null stream)
 
     ~ct_dmrg_s() {
         if (h) (void)hipStreamSynchronize(h->stream);
+        if (gstream) (void)hipStreamSynchronize(gstream);
+        for (BondGraph& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (gstream) (void)hipStreamDestroy(gstream);
         for (auto* v : {&W, &W2, &A, &Lenv, &Renv})
             for (double* q : *v)
                 if (q) (void)hipFree(q);
@@ -81,6 +102,80 @@ int build_envs(ct_dmrg_s* s) {
     return 0;
 }
 
+// H_eff preparation (both precisions) and the Lanczos solve of bond i,
+// enqueued on h->stream without any host synchronisation.
+int enqueue_local_solve(ct_dmrg_s* s, int i, const ct::HeffDims& hd) {
+    ct::Handle* h = s->h;
+    ct::HeffEnv env, env64;
+    CT_CHECK(ct::heff_prepare(h, s->p.prec, CT_F64, s->Lenv[i], s->Renv[i + 2], s->W2[i], hd, &env));
+    CT_CHECK(ct::heff_prepare(h, CT_PREC_FP64, CT_F64, s->Lenv[i], s->Renv[i + 2], s->W2[i], hd, &env64));
+    return ct::lanczos_enqueue(h, env, &env64, s->theta, s->p.max_krylov_iter, s->p.krylov_tol, s->p.jacobi_max_rot,
+                               s->p.refine);
+}
+
+// The local solve, launched kernel by kernel the first time a bond's key is
+// seen, captured into a graph the second time and replayed afterwards (at a
+// fixed bond dimension every sweep repeats every key).  A capture that fails
+// for any reason falls back to launching the kernels.
+int local_solve(ct_dmrg_s* s, int i, const ct::HeffDims& hd, ct::LanczosResult* lr) {
+    ct::Handle* h = s->h;
+    if (!s->p.graphs) {
+        CT_CHECK(enqueue_local_solve(s, i, hd));
+        return ct::lanczos_finish(h, s->p.refine, true, lr);
+    }
+    std::array<int64_t, 16> key{};
+    double tol = s->p.krylov_tol;
+    int64_t tol_bits;
+    std::memcpy(&tol_bits, &tol, sizeof tol_bits);
+    key = {hd.Dl, hd.Dr, hd.wL, hd.wM, hd.wR, s->d, s->p.prec, s->p.refine, s->p.max_krylov_iter,
+           s->p.jacobi_max_rot, tol_bits, (int64_t)(uintptr_t)s->theta, (int64_t)(uintptr_t)s->Lenv[i],
+           (int64_t)(uintptr_t)s->Renv[i + 2], (int64_t)(uintptr_t)s->W2[i], 0};
+    BondGraph& g = s->graphs[i];
+    const bool replay = g.exec && g.exec_key == key && g.gen == h->ws_gen;
+    if (!replay) {
+        if (g.no_capture || !g.seen || g.seen_key != key) {
+            CT_CHECK(enqueue_local_solve(s, i, hd));
+            if (!g.no_capture) {
+                g.seen_key = key;
+                g.seen = true;
+            }
+            return ct::lanczos_finish(h, s->p.refine, true, lr);
+        }
+        // capture (the eager run with this key sized every workspace)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        hipStream_t user = h->stream;
+        hipGraph_t graph = nullptr;
+        int rc = ct::hip_rc(hipStreamBeginCapture(s->gstream, hipStreamCaptureModeRelaxed));
+        if (rc == 0) {
+            h->stream = s->gstream;
+            rc = enqueue_local_solve(s, i, hd);
+            h->stream = user;
+            const hipError_t e = hipStreamEndCapture(s->gstream, &graph);
+            if (rc == 0 && e != hipSuccess) rc = ct::hip_rc(e);
+        }
+        if (rc == 0 && graph) rc = ct::hip_rc(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != 0 || !g.exec) {
+            if (s->p.verbose)
+                fprintf(stderr, "bond %d: graph capture failed (status %d: %s); launching its kernels instead\n", i, rc,
+                        rc != 0 ? ct_last_error() : "no graph");
+            (void)hipGetLastError();
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+            g.no_capture = true;
+            CT_CHECK(enqueue_local_solve(s, i, hd));
+            return ct::lanczos_finish(h, s->p.refine, true, lr);
+        }
+        g.exec_key = key;
+        g.gen = h->ws_gen;
+    }
+    // replay on the caller's stream: ordered after theta, before the truncation
+    CT_HIP(hipGraphLaunch(g.exec, h->stream));
+    s->st.graph_replays += 1;
+    return ct::lanczos_finish(h, s->p.refine, true, lr);
+}
+
 int local_update(ct_dmrg_s* s, int i, bool l2r, double* E_out) {
     ct::Handle* h = s->h;
     const int64_t d = s->d;
@@ -98,12 +193,8 @@ int local_update(ct_dmrg_s* s, int i, bool l2r, double* E_out) {
     }
     double t1 = now_s();
     ct::HeffDims hd{Dl, Dl, d, Dr, Dr, wL, wM, wR};
-    ct::HeffEnv env, env64;
-    CT_CHECK(ct::heff_prepare(h, s->p.prec, CT_F64, s->Lenv[i], s->Renv[i + 2], s->W2[i], hd, &env));
-    CT_CHECK(ct::heff_prepare(h, CT_PREC_FP64, CT_F64, s->Lenv[i], s->Renv[i + 2], s->W2[i], hd, &env64));
     ct::LanczosResult lr;
-    CT_CHECK(ct::lanczos(h, env, &env64, s->theta, s->p.max_krylov_iter, s->p.krylov_tol, s->p.jacobi_max_rot,
-                         s->p.refine, &lr));
+    CT_CHECK(local_solve(s, i, hd, &lr));
     double t2 = now_s();
     // split_two_site_theta_svd (dmrg.mojo:143-252): A_i = U (S) , A_{i+1} = (S) Vt
     ct::SvdResult sr;
@@ -184,12 +275,16 @@ int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int6
         for (int l = N - 1; l >= 0; --l) cr[l] = std::max(s->D[l], grow(cr[l + 1]));
         for (int l = 0; l <= N; ++l) s->Dcap[l] = std::max(s->D[l], std::min(cl[l], cr[l]));
     }
+    s->graphs.assign(N, BondGraph());
     s->W.assign(N, nullptr);
     s->W2.assign(N, nullptr);
     s->A.assign(N, nullptr);
     s->Lenv.assign(N + 1, nullptr);
     s->Renv.assign(N + 1, nullptr);
     int rc = 0;
+    if (params->graphs) {  // default priority, like every stream of the library
+        rc = ct::hip_rc(hipStreamCreateWithFlags(&s->gstream, hipStreamNonBlocking));
+    }
     int64_t theta_cap = 1, s_cap = 1;
     for (int i = 0; i < N && rc == 0; ++i) {
         const int64_t nw = s->w[i] * d * d * s->w[i + 1];

@@ -72,6 +72,9 @@ struct Handle {
     double* d_scalars = nullptr;  // small FP64 scratch for reductions (pinned-size)
     double* h_pinned = nullptr;   // pinned host mirror for scalar read-back
     int num_cu = 256;
+    // bumped whenever a workspace is reallocated: device pointers recorded
+    // before (e.g. in a captured graph) are stale once it changes
+    uint64_t ws_gen = 0;
 };
 
 enum WsSlot {
@@ -261,6 +264,13 @@ struct LanczosResult {
 // for the refinement matvec (only used when refine != 0).
 int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta, int max_iter, double tol,
             int max_rot, int refine, LanczosResult* res);
+// The same solve split in two: lanczos_enqueue only enqueues kernels on
+// h->stream (no allocation once the shapes have been seen, no host
+// synchronisation, so it can be captured into a graph); lanczos_finish reads
+// the device state back (synchronises) and fills *res.
+int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta, int max_iter, double tol,
+            int max_rot, int refine);
+int lanczos_finish(Handle* h, int refine, bool have_env64, LanczosResult* res);
 
 // SVD / truncation --------------------------------------------------------
 struct SvdResult {

@@ -498,8 +498,8 @@ inline int grid_n(int64_t n) {
 
 }  // namespace
 
-int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta, int max_iter, double tol,
-            int max_rot, int refine, LanczosResult* res) {
+int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta, int max_iter, double tol,
+                    int max_rot, int refine) {
     const HeffDims& D = env.dims;
     const int64_t dim = D.Dl * D.d * D.d * D.Dr;
     CT_ARG(max_iter >= 1 && max_iter <= 64, 5, "max_iter must be in [1, 64]");
@@ -571,10 +571,16 @@ int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta
         }
         CT_HIP(hipGetLastError());
     }
+    return 0;
+}
 
+int lanczos_finish(Handle* h, int refine, bool have_env64, LanczosResult* res) {
+    void* sb;
+    CT_CHECK(ws_get(h, WS_LZ, sizeof(LzState), &sb));
+    const LzState* s = (const LzState*)sb;
     LzState* hs = (LzState*)h->h_pinned;
-    CT_HIP(hipMemcpyAsync(hs, s, sizeof(LzState), hipMemcpyDeviceToHost, st));
-    CT_HIP(hipStreamSynchronize(st));
+    CT_HIP(hipMemcpyAsync(hs, s, sizeof(LzState), hipMemcpyDeviceToHost, h->stream));
+    CT_HIP(hipStreamSynchronize(h->stream));
     if (hs->err == 1) {
         set_last_error("Initial vector has zero norm");
         return CT_STATUS_ZERO_NORM;
@@ -588,8 +594,14 @@ int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta
     res->E_rq = hs->scal[3];
     res->res_norm = sqrt(hs->scal[4]);
     res->E0 = res->E_lanczos;
-    if (refine >= 1 && env64) res->E0 = refine >= 2 ? hs->scal[6] : hs->scal[3];
+    if (refine >= 1 && have_env64) res->E0 = refine >= 2 ? hs->scal[6] : hs->scal[3];
     return 0;
+}
+
+int lanczos(Handle* h, const HeffEnv& env, const HeffEnv* env64, double* d_theta, int max_iter, double tol,
+            int max_rot, int refine, LanczosResult* res) {
+    CT_CHECK(lanczos_enqueue(h, env, env64, d_theta, max_iter, tol, max_rot, refine));
+    return lanczos_finish(h, refine, env64 != nullptr, res);
 }
 
 }  // namespace ct

@@ -40,6 +40,9 @@ class DMRGParams:
                     give the same kept subspace and discarded weight; the Gram
                     form resolves singular vectors below ~1e-8 sigma_max less
                     accurately, which moves energies by far less than 1e-10.
+    graphs          replay each bond's Lanczos solve from a HIP graph cached per
+                    bond once its shapes repeat (identical results; removes the
+                    per-kernel launch cost that dominates at small bond dimensions)
     """
     num_sweeps: int
     chi_max: int
@@ -54,6 +57,7 @@ class DMRGParams:
     env_prec: int = PREC_FP64
     jacobi_max_rot: int = 200
     svd_mode: int = SVD_GRAM
+    graphs: bool = True
 
 
 @dataclass
@@ -72,6 +76,7 @@ class DMRGStats:
     matvecs: int = 0
     matvec_flops: float = 0.0
     env_flops: float = 0.0
+    graph_replays: int = 0
     sweep_energies: list = field(default_factory=list)
 
 
@@ -252,7 +257,7 @@ class DMRGRun:
         self._mps = [s.tensor.storage.to(torch.float64) for s in mps.sites]
         p = _lib.DMRGParamsC(params.num_sweeps, params.chi_max, params.eps_trunc, params.max_krylov_iter,
                              params.krylov_tol, params.energy_tol, params.jacobi_max_rot, params.svd_mode, params.prec,
-                             params.refine, params.env_prec, int(params.verbose))
+                             params.refine, params.env_prec, int(params.verbose), int(params.graphs))
         wd = _lib.i64_array(mpo.bond_dims)
         md = _lib.i64_array(mps.bond_dims)
         wp = (ctypes.c_void_p * N)(*[t.data_ptr() for t in self._mpo])

@@ -224,6 +224,11 @@ typedef struct {
     int refine;            /* 0: reference energies; 1: FP64 Rayleigh quotient; 2: + residual step */
     int env_prec;          /* precision of the environment updates (CT_PREC_*) */
     int verbose;
+    int graphs;            /* 1: replay each bond's Lanczos solve (H_eff preparation, Krylov recurrence,
+                              tridiagonal solve, Ritz vector, refinement) from a HIP graph cached per bond,
+                              captured the second time the bond's shapes repeat; 0: launch every kernel.
+                              Same kernels and arguments either way, so the results are identical.
+                              No counterpart in the reference (launch overhead at small bond dimensions). */
 } ct_dmrg_params;
 
 typedef struct {
@@ -241,6 +246,7 @@ typedef struct {
     int64_t matvecs;           /* number of H_eff applications */
     double matvec_flops;       /* algorithmic FLOPs of those matvecs (reference order) */
     double env_flops;
+    int64_t graph_replays;     /* local solves replayed from a cached graph (ct_dmrg_params.graphs) */
 } ct_dmrg_stats;
 
 /* mpo_dims[nsites+1], mpo_sites[i]: device FP64 [w_i,d,d,w_{i+1}];

@@ -622,3 +622,47 @@ def test_dmrg_quantum_chemistry_number_sector_start(ctx, prec, refine):
     psi = psi.reshape(-1)
     leak = float(psi @ (ham.mpo_to_dense(pen_t) @ psi) / (psi @ psi))
     assert leak < 1e-10, leak
+
+
+@pytest.mark.parametrize("prec,refine", [(ct.PREC_FP64, 0), (ct.PREC_BF16X3, 2)])
+def test_dmrg_graph_replay_matches_eager(ctx, prec, refine):
+    """Each bond's Lanczos solve replayed from its cached HIP graph gives the same
+    sweeps, bit for bit, as launching every kernel.  The bond dimension is fixed
+    (eps_trunc = 0), so a bond's key repeats on every visit: the first visit runs
+    eagerly, the second captures, every later one replays, with the eager
+    truncations and environment updates of the other bonds in between."""
+    L = 12
+    mpo_t = ham.j1j2_tensors(L, 1.0, 0.5)
+    mps_t = ct.random_mps_c_compatible_tensors(L, 2, 16, 7)
+    runs = {}
+    for graphs in (False, True):
+        p = ct.DMRGParams(num_sweeps=4, chi_max=16, eps_trunc=0.0, max_krylov_iter=8, krylov_tol=1e-12,
+                          energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=-1, graphs=graphs)
+        E, out, st = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p,
+                                      return_stats=True)
+        runs[graphs] = (E, [s.tensor.to_numpy() for s in out.sites], st)
+    (E0, A0, st0), (E1, A1, st1) = runs[False], runs[True]
+    assert st0.graph_replays == 0
+    assert st1.graph_replays >= 2 * (L - 1) * (4 - 1)  # every visit after the first two of each bond
+    assert st1.sweep_energies == st0.sweep_energies and E1 == E0
+    assert all(np.array_equal(a, b) for a, b in zip(A0, A1))
+    assert st1.matvecs == st0.matvecs
+
+
+def test_dmrg_graph_replay_growing_bonds_matches_eager(ctx):
+    """Bond dimensions that change between visits (L=16 XXZ from a D=8 start,
+    chi_max=32, eps_trunc=1e-10): a bond's key changes as its bonds grow, so
+    the graph cached for the old shapes must never be replayed for the new
+    ones.  Same sweeps, bit for bit, as launching every kernel."""
+    mpo_t = ham.heisenberg_xxz_tensors(16, 1.0, 1.0, 0.0)
+    mps_t = ct.random_mps_c_compatible_tensors(16, 2, 8, 42)
+    runs = {}
+    for graphs in (False, True):
+        p = ct.DMRGParams(num_sweeps=4, chi_max=32, eps_trunc=1e-10, max_krylov_iter=20, krylov_tol=1e-10,
+                          energy_tol=0.0, jacobi_max_rot=20000, graphs=graphs)
+        E, out, st = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p,
+                                      return_stats=True)
+        runs[graphs] = (E, st)
+    (E0, st0), (E1, st1) = runs[False], runs[True]
+    assert st1.graph_replays > 0
+    assert st1.sweep_energies == st0.sweep_energies and E1 == E0
