@@ -60,6 +60,7 @@ SIGNATURES = {
     "ct_scale": [_vp, _int, _dbl, _vp, _i64],
     "ct_convert": [_vp, _int, _vp, _int, _vp, _i64],
     "ct_heff_apply": [_vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp] + [_i64] * 8,
+    "ct_heff_apply_noalloc": [_vp, _int, _int, _vp, _vp, _vp, _vp, _vp, _vp] + [_i64] * 8,
     "ct_env_update_left": [_vp, _int, _int, _vp, _vp, _vp, _vp] + [_i64] * 5,
     "ct_env_update_right": [_vp, _int, _int, _vp, _vp, _vp, _vp] + [_i64] * 5,
     "ct_two_site_theta": [_vp, _int, _int, _vp, _vp, _vp] + [_i64] * 4,

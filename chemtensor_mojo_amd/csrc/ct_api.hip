@@ -12,6 +12,14 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 int buf_get(Handle* h, Buffer& b, size_t bytes, void** out) {
     if (bytes == 0) bytes = 16;
+    if (h->arena) {  // an allocation-free call never grows a buffer
+        if (b.bytes >= bytes) {
+            *out = b.ptr;
+            return 0;
+        }
+        set_last_error("allocation-free call: a buffer would have to grow");
+        return CT_STATUS_ALLOC;
+    }
     if (b.bytes < bytes) {
         // never inside a stream capture: the synchronisation and the new
         // address would break the graph being recorded
@@ -47,7 +55,30 @@ void buf_free(Buffer& b) {
     b.bytes = 0;
 }
 
-int ws_get(Handle* h, int slot, size_t bytes, void** out) { return buf_get(h, h->ws[slot], bytes, out); }
+bool arena_reserve(Arena& a, int slot, size_t bytes) {
+    const size_t start = (a.used + 255) & ~(size_t)255;
+    if (bytes == 0) bytes = 16;
+    if (start > a.cap || bytes > a.cap - start) return false;
+    a.off[slot] = start;
+    a.len[slot] = bytes;
+    a.used = start + bytes;
+    return true;
+}
+
+int ws_get(Handle* h, int slot, size_t bytes, void** out) {
+    if (h->arena) {
+        Arena& a = *h->arena;
+        if (bytes == 0) bytes = 16;
+        if (a.len[slot] < bytes) {
+            set_last_error("allocation-free call: workspace slot " + std::to_string(slot) + " needs " +
+                           std::to_string(bytes) + " bytes, " + std::to_string(a.len[slot]) + " were planned");
+            return CT_STATUS_ALLOC;
+        }
+        *out = a.base + a.off[slot];
+        return 0;
+    }
+    return buf_get(h, h->ws[slot], bytes, out);
+}
 
 }  // namespace ct
 
@@ -77,8 +108,14 @@ int ct_handle_create(int device, void* stream, ct_handle_t* out) {
         return CT_STATUS_ROCBLAS_ERROR_BASE;
     }
     rocblas_set_stream(h->blas, h->stream);
-    CT_HIP(hipMalloc(&h->d_scalars, 4096));
-    CT_HIP(hipHostMalloc(&h->h_pinned, kPinnedBytes, hipHostMallocDefault));
+    int rc = ct::hip_rc(hipMalloc(&h->d_scalars, 4096));
+    if (rc == 0) rc = ct::hip_rc(hipHostMalloc(&h->h_pinned, kPinnedBytes, hipHostMallocDefault));
+    // the allocation-free entry point's scratch array on this device (once per device)
+    if (rc == 0) rc = ct::noalloc_scratch_init(device);
+    if (rc != 0) {
+        (void)ct_handle_destroy(h);
+        return rc;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->num_cu = prop.multiProcessorCount;
     *out = h;

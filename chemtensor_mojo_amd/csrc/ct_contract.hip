@@ -202,26 +202,38 @@ int heff_apply(Handle* h, const HeffEnv& env, int tVec, const void* theta, void*
     CT_CHECK(ws_get(h, WS_T1, (size_t)nT1 * es, &t1));
     CT_CHECK(ws_get(h, WS_T2, (size_t)nU * es, &u));
     {   // step 1: T1[b][m=Dlb][n=(d d Dr)] = sum_k L[b][k][m] theta[k][n]
-        GemmDesc g;
-        g.M = (int)D.Dlb; g.N = (int)(d2 * D.Dr); g.K = (int)D.Dl; g.batch = (int)D.wL;
-        g.a_kc = 0; g.sAm = 1; g.sAk = D.Dlb; g.bsA = D.Dl * D.Dlb;
-        g.b_kc = 0; g.sBk = d2 * D.Dr; g.sBn = 1; g.bsB = 0;
-        g.ldc = d2 * D.Dr; g.bsC = D.Dlb * d2 * D.Dr;
+        GemmDesc g = heff_step1_desc(D);
         g.active = active;
         CT_CHECK(gemm(h, env.prec, env.tEnv, env.L, tVec, theta, tMid, t1, g));
     }
     // steps 2+3: fused MPO mixing
     CT_CHECK(mix(h, tMid, t1, tMid, u, env.W2, m, env.W2pad));
     {   // step 4: out[(Dlb d d)][Drb] = U[(Dlb d d)][(wR Dr)] R[(wR Dr)][Drb]
-        GemmDesc g;
-        g.M = (int)(D.Dlb * d2); g.N = (int)D.Drb; g.K = (int)(D.wR * D.Dr); g.batch = 1;
-        g.a_kc = 1; g.sAm = D.wR * D.Dr; g.sAk = 1;
-        g.b_kc = 0; g.sBk = D.Drb; g.sBn = 1;
-        g.ldc = D.Drb;
+        GemmDesc g = heff_step4_desc(D);
         g.active = active;
         CT_CHECK(gemm(h, env.prec, tMid, u, env.tEnv, env.R, tVec, out, g));
     }
     return 0;
+}
+
+GemmDesc heff_step1_desc(const HeffDims& D) {
+    const int64_t d2 = D.d * D.d;
+    GemmDesc g;
+    g.M = (int)D.Dlb; g.N = (int)(d2 * D.Dr); g.K = (int)D.Dl; g.batch = (int)D.wL;
+    g.a_kc = 0; g.sAm = 1; g.sAk = D.Dlb; g.bsA = D.Dl * D.Dlb;
+    g.b_kc = 0; g.sBk = d2 * D.Dr; g.sBn = 1; g.bsB = 0;
+    g.ldc = d2 * D.Dr; g.bsC = D.Dlb * d2 * D.Dr;
+    return g;
+}
+
+GemmDesc heff_step4_desc(const HeffDims& D) {
+    const int64_t d2 = D.d * D.d;
+    GemmDesc g;
+    g.M = (int)(D.Dlb * d2); g.N = (int)D.Drb; g.K = (int)(D.wR * D.Dr); g.batch = 1;
+    g.a_kc = 1; g.sAm = D.wR * D.Dr; g.sAk = 1;
+    g.b_kc = 0; g.sBk = D.Drb; g.sBn = 1;
+    g.ldc = D.Drb;
+    return g;
 }
 
 // ------------------------------------------------------------ environments

@@ -292,6 +292,36 @@ bool aligned16(const void* ptr, int64_t s1, int64_t s2) {
     return ((uintptr_t)ptr % 16 == 0) && (s1 % q == 0) && (s2 % q == 0);
 }
 
+// Split K when the tiles alone would leave most CUs idle (small bond
+// dimensions): at least 8 K-tiles per split, aiming at >= 256 workgroups.
+// With long K, also when they give fewer than two workgroups per CU: a
+// lone workgroup has nothing to run while it waits at its per-step
+// barrier (the [4096 x 8192] . [8192 x 1024] product of the D = 1024
+// matvec is 256 tiles of 128 x 128).
+void plan_splits(const GemmDesc& g, int BT, int* splits_out, int* kchunk_out) {
+    const int64_t tiles = (int64_t)((g.M + BT - 1) / BT) * ((g.N + BT - 1) / BT) * g.batch;
+    int splits = 1;
+    if (tiles < 128 && g.K >= 16 * BK) splits = (int)std::min<int64_t>((256 + tiles - 1) / tiles, g.K / (8 * BK));
+    else if (tiles < 512 && g.K >= 128 * BK) splits = (int)std::min<int64_t>((512 + tiles - 1) / tiles, g.K / (64 * BK));
+    int kchunk = g.K;
+    if (splits > 1) {
+        kchunk = (int)(((g.K + splits - 1) / splits + BK - 1) / BK * BK);
+        splits = (g.K + kchunk - 1) / kchunk;
+    }
+    *splits_out = splits;
+    *kchunk_out = kchunk;
+}
+
+// Tile size of the FP64/FP32 GEMM: 64x64 when 128x128 tiles would occupy
+// fewer than half the CUs and split-K cannot make up for it (K < 2048, or
+// fewer than 16 tiles): four times the workgroups for the same problem.  With
+// long K the 128x128 tiles split over K fill the GPU at a better
+// MFMA-to-load ratio (measured at D = 512).
+int plan_tile(const GemmDesc& g) {
+    const int64_t tiles128 = (int64_t)((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch;
+    return (tiles128 < 128 && (g.K < 2048 || tiles128 < 16)) ? 64 : 128;
+}
+
 template <typename TC, typename TA, typename TB, typename TO, int BT>
 int launch_bt(Handle* h, const void* A, const void* B, void* C, const GemmDesc& g) {
     KArgs p;
@@ -306,21 +336,8 @@ int launch_bt(Handle* h, const void* A, const void* B, void* C, const GemmDesc& 
     const bool bkc = g.b_kc != 0;
     p.vecA = akc ? aligned16<TA>(A, g.sAm, g.bsA) : aligned16<TA>(A, g.sAk, g.bsA);
     p.vecB = bkc ? aligned16<TB>(B, g.sBn, g.bsB) : aligned16<TB>(B, g.sBk, g.bsB);
-    // Split K when the tiles alone would leave most CUs idle (small bond
-    // dimensions): at least 8 K-tiles per split, aiming at >= 256 workgroups.
-    // With long K, also when they give fewer than two workgroups per CU: a
-    // lone workgroup has nothing to run while it waits at its per-step
-    // barrier (the [4096 x 8192] . [8192 x 1024] product of the D = 1024
-    // matvec is 256 tiles of 128 x 128).
-    const int64_t tiles = (int64_t)p.tilesM * p.tilesN * g.batch;
     int splits = 1;
-    if (tiles < 128 && g.K >= 16 * BK) splits = (int)std::min<int64_t>((256 + tiles - 1) / tiles, g.K / (8 * BK));
-    else if (tiles < 512 && g.K >= 128 * BK) splits = (int)std::min<int64_t>((512 + tiles - 1) / tiles, g.K / (64 * BK));
-    p.kchunk = g.K;
-    if (splits > 1) {
-        p.kchunk = (int)(((g.K + splits - 1) / splits + BK - 1) / BK * BK);
-        splits = (g.K + p.kchunk - 1) / p.kchunk;
-    }
+    plan_splits(g, BT, &splits, &p.kchunk);
     p.splits = splits;
     p.part = nullptr;
     if (splits > 1) CT_CHECK(ws_get(h, WS_SPLITK, (size_t)g.batch * splits * g.M * g.N * sizeof(TC), &p.part));
@@ -351,14 +368,9 @@ int launch_bt(Handle* h, const void* A, const void* B, void* C, const GemmDesc& 
     return 0;
 }
 
-// 64x64 tiles when 128x128 tiles would occupy fewer than half the CUs and
-// split-K cannot make up for it (K < 2048, or fewer than 16 tiles): four times
-// the workgroups for the same problem.  With long K the 128x128 tiles split
-// over K fill the GPU at a better MFMA-to-load ratio (measured at D = 512).
 template <typename TC, typename TA, typename TB, typename TO>
 int launch_typed(Handle* h, const void* A, const void* B, void* C, const GemmDesc& g) {
-    const int64_t tiles128 = (int64_t)((g.M + 127) / 128) * ((g.N + 127) / 128) * g.batch;
-    if (tiles128 < 128 && (g.K < 2048 || tiles128 < 16)) return launch_bt<TC, TA, TB, TO, 64>(h, A, B, C, g);
+    if (plan_tile(g) == 64) return launch_bt<TC, TA, TB, TO, 64>(h, A, B, C, g);
     return launch_bt<TC, TA, TB, TO, 128>(h, A, B, C, g);
 }
 
@@ -410,6 +422,15 @@ int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int 
     if (prec == CT_PREC_BF16X3) return gemm_bf16x3(h, tA, A, tB, B, tC, C, g);
     set_last_error("unknown precision");
     return CT_STATUS_NOT_SUPPORTED;
+}
+
+size_t gemm_splitk_bytes(int prec, const GemmDesc& g) {
+    if (g.M <= 0 || g.N <= 0 || g.batch <= 0 || g.K <= 0) return 0;
+    int splits = 1, kchunk = 0;
+    plan_splits(g, plan_tile(g), &splits, &kchunk);
+    if (splits <= 1) return 0;
+    const size_t es = prec == CT_PREC_FP64 ? 8 : 4;
+    return (size_t)g.batch * splits * g.M * g.N * es;
 }
 
 }  // namespace ct

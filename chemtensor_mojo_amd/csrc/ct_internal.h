@@ -62,6 +62,21 @@ struct Buffer {
     size_t bytes = 0;
 };
 
+// Workspaces carved from a fixed device region instead of hipMalloc: while a
+// handle's `arena` is set, ws_get returns the region reserved for the slot
+// (planned before any launch) and never allocates; a request beyond the
+// reservation is an error.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    size_t off[40] = {};
+    size_t len[40] = {};
+};
+// Reserves `bytes` (256-byte aligned) for `slot` in the arena; false when the
+// arena is full.
+bool arena_reserve(Arena& a, int slot, size_t bytes);
+
 struct Handle {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -69,6 +84,7 @@ struct Handle {
     rocblas_handle blas = nullptr;
     // named workspaces (H_eff intermediates, reductions, SVD, Lanczos basis ...)
     Buffer ws[40];
+    Arena* arena = nullptr;       // set only for the duration of an allocation-free call
     double* d_scalars = nullptr;  // small FP64 scratch for reductions (pinned-size)
     double* h_pinned = nullptr;   // pinned host mirror for scalar read-back
     int num_cu = 256;
@@ -140,6 +156,9 @@ struct GemmDesc {
 // dtype codes for operands: CT_F64 / CT_F32
 int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int tC, void* C,
          const GemmDesc& g);
+// Bytes of split-K partial sums (workspace WS_SPLITK) that gemm() requests for
+// g at precision CT_PREC_FP64 or CT_PREC_FP32 (0 when it does not split K).
+size_t gemm_splitk_bytes(int prec, const GemmDesc& g);
 
 // 3xBF16 operand in "split rows": row r of batch b starts at p + b*bs + r*ld;
 // for each 32-wide block kb of the contraction index it holds 32 hi = bf16(x)
@@ -239,6 +258,15 @@ int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, co
                  const HeffDims& hd, HeffEnv* env, HeffStore* store = nullptr);
 // out = H_eff theta.  tVec: element type of theta and out.
 int heff_apply(Handle* h, const HeffEnv& env, int tVec, const void* theta, void* out, const int* active);
+// The two GEMMs of heff_apply's FP64 / FP32 path: step 1 (T1 = L^T theta,
+// batched over wL) and step 4 (out = U R).
+GemmDesc heff_step1_desc(const HeffDims& D);
+GemmDesc heff_step4_desc(const HeffDims& D);
+
+// Resolves the device address of the allocation-free entry point's static
+// scratch array on `device` (which must be current); called once per device
+// by ct_handle_create.
+int noalloc_scratch_init(int device);
 
 int env_left(Handle* h, int prec, int dtype, const void* L, const void* A, const void* W, void* out,
              int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR);

@@ -52,6 +52,7 @@ enum {
     CT_STATUS_SVD_FAILED = 4,       /* rocSOLVER info != 0 (src/mylinalg/svd.mojo:75-76) */
     CT_STATUS_ALLOC = 5,
     CT_STATUS_SHAPE = 6,            /* "Dim mismatch" / wrong C shape (dense_tensor.mojo:1163-1167, 1294-1299) */
+    CT_STATUS_SCRATCH_LIMIT = 7,    /* an allocation-free call would need more than CT_NOALLOC_SCRATCH_BYTES */
     CT_STATUS_ROCBLAS_ERROR_BASE = 2000,
     CT_STATUS_HIP_ERROR_BASE = 1000
 };
@@ -134,6 +135,25 @@ int ct_heff_create(ct_handle_t h, int prec, int dtype, const void* L, const void
                    ct_heff_t* op);
 int ct_heff_apply_prepared(ct_heff_t op, const void* theta, void* out);
 int ct_heff_destroy(ct_heff_t op);
+
+/* Allocation-free H_eff matvec.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Arguments and
+ * result are exactly those of ct_heff_apply, for prec CT_PREC_FP64 or
+ * CT_PREC_FP32 and w*d*d <= 64 on both MPO bonds; the call performs no
+ * hipMalloc/hipFree and no allocating library routine.  Every intermediate
+ * lives in one static device array of CT_NOALLOC_SCRATCH_BYTES per device
+ * (held from the first ct_handle_create on that device until the process
+ * ends); dimensions whose intermediates exceed it are rejected before any
+ * work is enqueued (CT_STATUS_SCRATCH_LIMIT; about D <= 450 at w = 5 and
+ * D <= 350 at w = 8 in FP64).
+ * All calls on one device share that array: calls that overlap on one GPU
+ * (two streams, two threads, or two handles) race and must be serialised by
+ * the caller. */
+#define CT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ct_heff_apply_noalloc(ct_handle_t h, int prec, int dtype, const void* L, const void* Wi, const void* Wj,
+                          const void* R, const void* theta, void* out, int64_t Dl, int64_t Dlb, int64_t d,
+                          int64_t Dr, int64_t Drb, int64_t wL, int64_t wM, int64_t wR);
 
 /* update_left_environment (src/state/environments.mojo:99-201):
  *   out[wR,Dr,Dr'] = sum L[wL,Dl,Dl'] A[Dl,s,Dr] W[wL,s,s',wR] A[Dl',s',Dr']
