@@ -36,7 +36,7 @@ class DMRGStatsC(ctypes.Structure):
     _fields_ = [("energy", _dbl), ("sweeps", _int), ("local_updates", _int), ("trunc_weight_sum", _dbl),
                 ("trunc_weight_max", _dbl), ("max_bond", _i64), ("t_total", _dbl), ("t_lanczos", _dbl),
                 ("t_svd", _dbl), ("t_env", _dbl), ("t_theta", _dbl), ("matvecs", _i64),
-                ("matvec_flops", _dbl), ("env_flops", _dbl), ("graph_replays", _i64)]
+                ("matvec_flops", _dbl), ("env_flops", _dbl), ("graph_replays", _i64), ("kept_start", _i64)]
 
 
 # name -> argtypes (all return int status unless listed in _RESTYPES)
@@ -64,19 +64,19 @@ SIGNATURES = {
     "ct_env_update_left": [_vp, _int, _int, _vp, _vp, _vp, _vp] + [_i64] * 5,
     "ct_env_update_right": [_vp, _int, _int, _vp, _vp, _vp, _vp] + [_i64] * 5,
     "ct_two_site_theta": [_vp, _int, _int, _vp, _vp, _vp] + [_i64] * 4,
-    "ct_lanczos_two_site": [_vp, _int, _vp, _vp, _vp, _vp, _vp] + [_i64] * 6
+    "ct_lanczos_two_site": [_vp, _int, _int, _vp, _vp, _vp, _vp, _vp] + [_i64] * 6
     + [_int, _dbl, _int, _int, _dp, _ip, _dp, _dp],
-    "ct_svd_trunc": [_vp, _vp, _i64, _i64, _i64, _dbl, _vp, _vp, _vp, _i64p, _dp],
-    "ct_theta_split": [_vp, _int, _vp, _i64, _i64, _i64, _i64, _dbl, _int, _vp, _vp, _i64p, _dp],
+    "ct_svd_trunc": [_vp, _int, _vp, _i64, _i64, _i64, _dbl, _vp, _vp, _vp, _i64p, _dp],
+    "ct_theta_split": [_vp, _int, _int, _vp, _i64, _i64, _i64, _i64, _dbl, _int, _vp, _vp, _i64p, _dp],
     "ct_sytrd": [_vp, _i64, _vp, _vp, _vp, _vp],
     "ct_syevd": [_vp, _i64, _vp, _vp, _vp, _int],
-    "ct_dmrg_create": [_vp, ctypes.POINTER(DMRGParamsC), _int, _i64, _i64p, ctypes.POINTER(_vp), _i64p,
+    "ct_dmrg_create": [_vp, ctypes.POINTER(DMRGParamsC), _int, _i64, _int, _i64p, ctypes.POINTER(_vp), _i64p,
                        ctypes.POINTER(_vp), ctypes.POINTER(_vp)],
     "ct_dmrg_destroy": [_vp],
-    "ct_dmrg_run": [_vp, _int, _dp, _dp],
+    "ct_dmrg_run": [_vp, _int, _dp, _dp, _ip],
     "ct_dmrg_get_stats": [_vp, ctypes.POINTER(DMRGStatsC)],
     "ct_dmrg_get_dims": [_vp, _i64p],
-    "ct_dmrg_get_site": [_vp, _int, _vp],
+    "ct_dmrg_get_site": [_vp, _int, _int, _vp],
     "ct_heff_adjoint": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp] + [_i64] * 6,
     "ct_env_adjoint": [_vp, _vp, _vp, _vp, _vp] + [_i64] * 5,
     "ct_random_mps_c_compatible": [_int, _i64, _i64, ctypes.c_uint64, _i64p, _dp, _i64p],

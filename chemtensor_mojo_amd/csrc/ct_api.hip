@@ -1,5 +1,6 @@
 // extern "C" entry points of libdmrg_hip.so (declared in include/chemtensor_hip.h)
 // and the handle / workspace management behind them.
+#include <algorithm>
 #include <new>
 
 #include "ct_internal.h"
@@ -322,24 +323,52 @@ int ct_two_site_theta(ct_handle_t h, int prec, int dtype, const void* A_i, const
     return ct::gemm(h, prec, dtype, A_i, dtype, A_j, dtype, theta, g);
 }
 
-int ct_lanczos_two_site(ct_handle_t h, int prec, const double* L, const double* Wi, const double* Wj, const double* R,
-                        double* theta, int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wM, int64_t wR,
-                        int max_iter, double tol, int max_rotations, int refine, double* E0, int* num_iter,
-                        double* E_lanczos, double* residual_norm) {
+int ct_lanczos_two_site(ct_handle_t h, int prec, int dtype, const void* L, const void* Wi, const void* Wj,
+                        const void* R, void* theta, int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wM,
+                        int64_t wR, int max_iter, double tol, int max_rotations, int refine, double* E0,
+                        int* num_iter, double* E_lanczos, double* residual_norm) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
     CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
-    CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wM > 0 && wR > 0, 9, "dimensions must be > 0");
-    CT_ARG(max_iter >= 1 && max_iter <= 64, 15, "max_iter must be in [1, 64]");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
+    CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wM > 0 && wR > 0, 10, "dimensions must be > 0");
+    CT_ARG(max_iter >= 1 && max_iter <= 64, 16, "max_iter must be in [1, 64]");
+    CT_ARG(max_rotations >= -1, 18, "max_rotations must be >= -1");
+    const int64_t nL = wL * Dl * Dl, nR = wR * Dr * Dr, nWi = wL * d * d * wM, nWj = wM * d * d * wR;
+    const int64_t dim = Dl * d * d * Dr;
+    const double *L64 = (const double*)L, *R64 = (const double*)R, *Wi64 = (const double*)Wi,
+                 *Wj64 = (const double*)Wj;
+    double* th64 = (double*)theta;
+    if (dtype == CT_F32) {
+        // float32 in and out, FP64 inside (Krylov basis, tridiagonal solve, refinement)
+        void *bl, *br, *bw, *bt;
+        CT_CHECK(ct::ws_get(h, ct::WS_BND0, (size_t)nL * 8, &bl));
+        CT_CHECK(ct::ws_get(h, ct::WS_BND1, (size_t)nR * 8, &br));
+        CT_CHECK(ct::ws_get(h, ct::WS_BND2, (size_t)(nWi + nWj) * 8, &bw));
+        CT_CHECK(ct::ws_get(h, ct::WS_BND3, (size_t)dim * 8, &bt));
+        CT_CHECK(ct::convert(h, CT_F32, L, CT_F64, bl, nL));
+        CT_CHECK(ct::convert(h, CT_F32, R, CT_F64, br, nR));
+        CT_CHECK(ct::convert(h, CT_F32, Wi, CT_F64, bw, nWi));
+        CT_CHECK(ct::convert(h, CT_F32, Wj, CT_F64, (double*)bw + nWi, nWj));
+        CT_CHECK(ct::convert(h, CT_F32, theta, CT_F64, bt, dim));
+        L64 = (const double*)bl;
+        R64 = (const double*)br;
+        Wi64 = (const double*)bw;
+        Wj64 = (const double*)bw + nWi;
+        th64 = (double*)bt;
+    }
     ct::HeffDims hd{Dl, Dl, d, Dr, Dr, wL, wM, wR};
     void* w2;
     CT_CHECK(ct::ws_get(h, ct::WS_W2, (size_t)(wL * d * d * d * d * wR) * 8, &w2));
-    CT_CHECK(ct::heff_build_w2(h, CT_F64, Wi, Wj, wL, wM, wR, d, (double*)w2));
+    CT_CHECK(ct::heff_build_w2(h, CT_F64, Wi64, Wj64, wL, wM, wR, d, (double*)w2));
     ct::HeffEnv env, env64;
-    CT_CHECK(ct::heff_prepare(h, prec, CT_F64, L, R, (const double*)w2, hd, &env));
-    CT_CHECK(ct::heff_prepare(h, CT_PREC_FP64, CT_F64, L, R, (const double*)w2, hd, &env64));
+    CT_CHECK(ct::heff_prepare(h, prec, CT_F64, L64, R64, (const double*)w2, hd, &env));
+    CT_CHECK(ct::heff_prepare(h, CT_PREC_FP64, CT_F64, L64, R64, (const double*)w2, hd, &env64));
     ct::LanczosResult r;
-    CT_ARG(max_rotations >= -1, 17, "max_rotations must be >= -1");
-    CT_CHECK(ct::lanczos(h, env, &env64, theta, max_iter, tol, max_rotations, refine, &r));
+    CT_CHECK(ct::lanczos(h, env, &env64, th64, max_iter, tol, max_rotations, refine, &r));
+    if (dtype == CT_F32) {
+        CT_CHECK(ct::convert(h, CT_F64, th64, CT_F32, theta, dim));
+        CT_HIP(hipStreamSynchronize(h->stream));
+    }
     if (E0) *E0 = r.E0;
     if (num_iter) *num_iter = r.num_iter;
     if (E_lanczos) *E_lanczos = r.E_lanczos;
@@ -347,27 +376,61 @@ int ct_lanczos_two_site(ct_handle_t h, int prec, const double* L, const double* 
     return 0;
 }
 
-int ct_svd_trunc(ct_handle_t h, const double* A, int64_t m, int64_t n, int64_t chi_max, double eps, double* U,
-                 double* S, double* Vt, int64_t* chi, double* discarded_weight) {
+int ct_svd_trunc(ct_handle_t h, int dtype, const void* A, int64_t m, int64_t n, int64_t chi_max, double eps, void* U,
+                 void* S, void* Vt, int64_t* chi, double* discarded_weight) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
-    CT_ARG(m >= 0 && n >= 0, 3, "dimensions must be >= 0");
-    CT_ARG(chi_max >= 1, 5, "chi_max must be >= 1");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 2, "unknown dtype");
+    CT_ARG(m >= 0 && n >= 0, 4, "dimensions must be >= 0");
+    CT_ARG(chi_max >= 1, 6, "chi_max must be >= 1");
     ct::SvdResult r;
-    CT_CHECK(ct::svd_trunc(h, A, m, n, chi_max, eps, U, S, Vt, &r));
+    if (dtype == CT_F64) {
+        CT_CHECK(ct::svd_trunc(h, (const double*)A, m, n, chi_max, eps, (double*)U, (double*)S, (double*)Vt, &r));
+    } else {
+        // float32 in and out, the factorisation and truncation rule in FP64
+        const int64_t k = std::min(std::min(m, n), chi_max);
+        void *ba, *bu, *bs, *bv;
+        CT_CHECK(ct::ws_get(h, ct::WS_BND0, (size_t)(m * n) * 8, &ba));
+        CT_CHECK(ct::ws_get(h, ct::WS_BND1, (size_t)(m * k) * 8, &bu));
+        CT_CHECK(ct::ws_get(h, ct::WS_BND2, (size_t)k * 8, &bs));
+        CT_CHECK(ct::ws_get(h, ct::WS_BND3, (size_t)(k * n) * 8, &bv));
+        CT_CHECK(ct::convert(h, CT_F32, A, CT_F64, ba, m * n));
+        CT_CHECK(ct::svd_trunc(h, (const double*)ba, m, n, chi_max, eps, (double*)bu, (double*)bs, (double*)bv, &r));
+        CT_CHECK(ct::convert(h, CT_F64, bu, CT_F32, U, m * r.chi));
+        CT_CHECK(ct::convert(h, CT_F64, bs, CT_F32, S, r.chi));
+        CT_CHECK(ct::convert(h, CT_F64, bv, CT_F32, Vt, r.chi * n));
+        CT_HIP(hipStreamSynchronize(h->stream));
+    }
     if (chi) *chi = r.chi;
     if (discarded_weight) *discarded_weight = r.discarded;
     return 0;
 }
 
-int ct_theta_split(ct_handle_t h, int svd_mode, const double* theta, int64_t Dl, int64_t d, int64_t Dr,
-                   int64_t chi_max, double eps, int left_to_right, double* A_i, double* A_j, int64_t* chi,
+int ct_theta_split(ct_handle_t h, int svd_mode, int dtype, const void* theta, int64_t Dl, int64_t d, int64_t Dr,
+                   int64_t chi_max, double eps, int left_to_right, void* A_i, void* A_j, int64_t* chi,
                    double* discarded_weight) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
     CT_ARG(svd_mode == CT_SVD_GESVD || svd_mode == CT_SVD_GRAM, 2, "unknown svd_mode");
-    CT_ARG(Dl > 0 && d > 0 && Dr > 0, 4, "dimensions must be > 0");
-    CT_ARG(chi_max >= 1, 7, "chi_max must be >= 1");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
+    CT_ARG(Dl > 0 && d > 0 && Dr > 0, 5, "dimensions must be > 0");
+    CT_ARG(chi_max >= 1, 8, "chi_max must be >= 1");
+    const int64_t m = Dl * d, n = d * Dr;
     ct::SvdResult r;
-    CT_CHECK(ct::theta_split(h, svd_mode, theta, Dl * d, d * Dr, chi_max, eps, left_to_right != 0, A_i, A_j, &r));
+    if (dtype == CT_F64) {
+        CT_CHECK(ct::theta_split(h, svd_mode, (const double*)theta, m, n, chi_max, eps, left_to_right != 0,
+                                 (double*)A_i, (double*)A_j, &r));
+    } else {
+        const int64_t k = std::min(std::min(m, n), chi_max);
+        void *bt, *bi, *bj;
+        CT_CHECK(ct::ws_get(h, ct::WS_BND0, (size_t)(m * n) * 8, &bt));
+        CT_CHECK(ct::ws_get(h, ct::WS_BND1, (size_t)(m * k) * 8, &bi));
+        CT_CHECK(ct::ws_get(h, ct::WS_BND2, (size_t)(k * n) * 8, &bj));
+        CT_CHECK(ct::convert(h, CT_F32, theta, CT_F64, bt, m * n));
+        CT_CHECK(ct::theta_split(h, svd_mode, (const double*)bt, m, n, chi_max, eps, left_to_right != 0, (double*)bi,
+                                 (double*)bj, &r));
+        CT_CHECK(ct::convert(h, CT_F64, bi, CT_F32, A_i, m * r.chi));
+        CT_CHECK(ct::convert(h, CT_F64, bj, CT_F32, A_j, r.chi * n));
+        CT_HIP(hipStreamSynchronize(h->stream));
+    }
     if (chi) *chi = r.chi;
     if (discarded_weight) *discarded_weight = r.discarded;
     return 0;

@@ -83,6 +83,12 @@ double env_flops(int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR) {
     return 2.0 * wL * Dl * Dl * d * Dr + 2.0 * Dl * Dr * wL * d * d * wR + 2.0 * Dr * wR * Dl * d * Dr;
 }
 
+// FP64 copy of a caller's FP64 or float32 device buffer
+int copy_in(ct::Handle* h, int dtype, const void* src, double* dst, int64_t n) {
+    if (dtype == CT_F64) return ct::hip_rc(hipMemcpyAsync(dst, src, (size_t)n * 8, hipMemcpyDeviceToDevice, h->stream));
+    return ct::convert(h, CT_F32, src, CT_F64, dst, n);
+}
+
 int alloc_d(double** p, int64_t n) {
     if (n < 1) n = 1;
     CT_HIP(hipMalloc(p, (size_t)n * sizeof(double)));
@@ -223,9 +229,9 @@ int local_update(ct_dmrg_s* s, int i, bool l2r, double* E_out) {
     st.t_svd += t3 - t2;
     st.t_env += t4 - t3;
     st.local_updates += 1;
-    int nmv = lr.num_iter;  // Krylov matvecs
-    if (s->p.refine >= 1) nmv += (s->p.refine >= 2 ? 2 : 1);
+    const int nmv = lr.matvecs;  // Krylov matvecs + refinement
     st.matvecs += nmv;
+    st.kept_start += lr.kept_start;
     st.matvec_flops += nmv * matvec_flops(Dl, d, Dr, wL, wM, wR);
     st.env_flops += l2r ? env_flops(Dl, d, chi, wL, wM) : env_flops(chi, d, Dr, wM, wR);
     st.trunc_weight_sum += sr.discarded;
@@ -242,13 +248,14 @@ int local_update(ct_dmrg_s* s, int i, bool l2r, double* E_out) {
 
 extern "C" {
 
-int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int64_t d, const int64_t* mpo_dims,
-                   const double* const* mpo_sites, const int64_t* mps_dims, const double* const* mps_sites,
-                   ct_dmrg_t* out) {
+int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int64_t d, int dtype,
+                   const int64_t* mpo_dims, const void* const* mpo_sites, const int64_t* mps_dims,
+                   const void* const* mps_sites, ct_dmrg_t* out) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
     CT_ARG(params != nullptr, 2, "params is NULL");
     CT_ARG(nsites >= 2, 3, "DMRG requires at least 2 sites");
     CT_ARG(d >= 1, 4, "d must be >= 1");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 5, "unknown dtype");
     CT_ARG(params->chi_max >= 1, 2, "chi_max must be >= 1");
     CT_ARG(params->max_krylov_iter >= 1 && params->max_krylov_iter <= 64, 2, "max_krylov_iter must be in [1, 64]");
     CT_ARG(params->prec >= 0 && params->prec <= 2, 2, "unknown precision");
@@ -289,13 +296,9 @@ int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int6
     for (int i = 0; i < N && rc == 0; ++i) {
         const int64_t nw = s->w[i] * d * d * s->w[i + 1];
         rc = alloc_d(&s->W[i], nw);
-        if (rc == 0)
-            rc = ct::hip_rc(hipMemcpyAsync(s->W[i], mpo_sites[i], (size_t)nw * 8, hipMemcpyDeviceToDevice,
-                                                   h->stream));
+        if (rc == 0) rc = copy_in(h, dtype, mpo_sites[i], s->W[i], nw);
         if (rc == 0) rc = alloc_d(&s->A[i], s->Dcap[i] * d * s->Dcap[i + 1]);
-        if (rc == 0)
-            rc = ct::hip_rc(hipMemcpyAsync(s->A[i], mps_sites[i], (size_t)(s->D[i] * d * s->D[i + 1]) * 8,
-                                                   hipMemcpyDeviceToDevice, h->stream));
+        if (rc == 0) rc = copy_in(h, dtype, mps_sites[i], s->A[i], s->D[i] * d * s->D[i + 1]);
     }
     for (int l = 0; l <= N && rc == 0; ++l) {
         rc = alloc_d(&s->Lenv[l], s->w[l] * s->Dcap[l] * s->Dcap[l]);
@@ -326,11 +329,12 @@ int ct_dmrg_destroy(ct_dmrg_t s) {
     return 0;
 }
 
-int ct_dmrg_run(ct_dmrg_t s, int num_sweeps, double* energy, double* sweep_energies) {
+int ct_dmrg_run(ct_dmrg_t s, int num_sweeps, double* energy, double* sweep_energies, int* sweeps_run) {
     CT_ARG(s != nullptr, 1, "state is NULL");
     const int sweeps = num_sweeps > 0 ? num_sweeps : s->p.num_sweeps;
     const int N = s->N;
     const double t0 = now_s();
+    if (sweeps_run) *sweeps_run = 0;
     CT_CHECK(build_envs(s));
     double E = 0.0, prev = 0.0;
     for (int sw = 0; sw < sweeps; ++sw) {
@@ -339,6 +343,7 @@ int ct_dmrg_run(ct_dmrg_t s, int num_sweeps, double* energy, double* sweep_energ
         if (sweep_energies) sweep_energies[sw] = E;
         s->sweep_E.push_back(E);
         s->st.sweeps += 1;
+        if (sweeps_run) *sweeps_run = sw + 1;
         if (s->p.verbose) fprintf(stderr, "Sweep %d completed. Energy: %.15g\n", sw + 1, E);
         if (sw > 0 && std::fabs(E - prev) < s->p.energy_tol) break;
         prev = E;
@@ -364,11 +369,15 @@ int ct_dmrg_get_dims(ct_dmrg_t s, int64_t* dims) {
     return 0;
 }
 
-int ct_dmrg_get_site(ct_dmrg_t s, int i, double* dst) {
+int ct_dmrg_get_site(ct_dmrg_t s, int i, int dtype, void* dst) {
     CT_ARG(s != nullptr, 1, "state is NULL");
     CT_ARG(i >= 0 && i < s->N, 2, "site index out of range");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
     const int64_t n = s->D[i] * s->d * s->D[i + 1];
-    CT_HIP(hipMemcpyAsync(dst, s->A[i], (size_t)n * 8, hipMemcpyDeviceToDevice, s->h->stream));
+    if (dtype == CT_F64)
+        CT_HIP(hipMemcpyAsync(dst, s->A[i], (size_t)n * 8, hipMemcpyDeviceToDevice, s->h->stream));
+    else
+        CT_CHECK(ct::convert(s->h, CT_F64, s->A[i], CT_F32, dst, n));
     CT_HIP(hipStreamSynchronize(s->h->stream));
     return 0;
 }

@@ -129,7 +129,11 @@ enum WsSlot {
     WS_MIXA = 32,    // large-w mixing: packed input
     WS_MIXB = 33,    // large-w mixing: GEMM output
     WS_MIXC = 34,    // large-w mixing: split coefficients (3xBF16)
-    WS_COUNT = 35
+    WS_BND0 = 35,    // FP64 copies of float32 arguments at the C-ABI boundary
+    WS_BND1 = 36,
+    WS_BND2 = 37,
+    WS_BND3 = 38,
+    WS_COUNT = 39
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);
@@ -287,6 +291,8 @@ struct LanczosResult {
     double E_rq = 0.0;      // FP64 Rayleigh quotient <psi|H|psi> (refinement)
     double res_norm = 0.0;  // ||H psi - E_rq psi|| (refinement)
     int num_iter = 0;
+    int matvecs = 0;        // H_eff applications (Krylov + refinement)
+    int kept_start = 0;     // the refined result lay above the start vector's energy: v0 returned
 };
 // env: environments prepared for the Krylov matvecs; env64: FP64 environments
 // for the refinement matvec (only used when refine != 0).

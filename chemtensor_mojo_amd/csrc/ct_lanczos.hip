@@ -21,6 +21,14 @@
 //   refine >= 2: one Rayleigh-Ritz step in span{psi, r/|r|}, whose 2x2
 //                projected matrix [[E_rq, |r|], [|r|, <r^|H r^>]] uses one more
 //                matvec at the Krylov precision; psi and the energy are updated.
+// With refinement the first Krylov matvec (H v0) also runs in FP64, so
+// alpha_0 = <v0|H v0> is the start vector's energy to FP64 rounding.  A
+// mixed-precision Krylov space can yield a Ritz vector whose true energy lies
+// above its start's (the recurrence coefficients carry the matvec error);
+// when the refined energy exceeds alpha_0 the solve returns v0 with energy
+// alpha_0 instead, so no local update raises the energy.  When the Krylov
+// space breaks down after one vector the Ritz vector is v0 itself and the
+// saved FP64 product H v0 replaces the refinement's FP64 matvec.
 #include "ct_internal.h"
 #include "ct_device.h"
 
@@ -38,15 +46,21 @@ struct LzState {
     int active;
     int numiter;
     int err;
-    int pad;
+    int exact_start;  // H v0 ran in FP64 (refinement of a mixed-precision solve)
+    int refmv;        // the refinement's FP64 matvec of psi is needed (numiter > 1)
+    int kept_start;   // the refined result lay above alpha_0: v0 was returned
+    int pad[2];
 };
 
 constexpr int NB = 512;  // reduction blocks
 
-__global__ void lz_init_kernel(LzState* s, int max_iter) {
+__global__ void lz_init_kernel(LzState* s, int max_iter, int exact_start) {
     s->active = 1;
     s->numiter = max_iter;
     s->err = 0;
+    s->exact_start = exact_start;
+    s->refmv = 1;
+    s->kept_start = 0;
     for (int i = 0; i < MAXK; ++i) { s->alpha[i] = 0.0; s->beta[i] = 0.0; s->coef[i] = 0.0; }
     for (int i = 0; i < 16; ++i) s->scal[i] = 0.0;
 }
@@ -491,6 +505,65 @@ __global__ __launch_bounds__(256) void lz_correct(double* __restrict__ psi, cons
         psi[i] = c0 * psi[i] + c1 * rhat[i];
 }
 
+// dst = src (unless the start vector was zero)
+__global__ __launch_bounds__(256) void lz_copy(const double* __restrict__ src, double* __restrict__ dst, int64_t n,
+                                               const LzState* s) {
+    if (s->err) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// One Krylov vector (breakdown at the first beta): the Ritz vector is v0, and
+// H psi is the saved FP64 product H v0; otherwise the refinement's FP64 matvec
+// of psi runs (refmv).
+__global__ __launch_bounds__(256) void lz_take_start(const double* __restrict__ v0, const double* __restrict__ hv0,
+                                                     double* __restrict__ psi, double* __restrict__ w, int64_t n,
+                                                     LzState* s) {
+    if (s->err) return;
+    const bool one = s->numiter == 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) s->refmv = one ? 0 : 1;
+    if (!one) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        psi[i] = v0[i];
+        w[i] = hv0[i];
+    }
+}
+
+// Refined energy above the start vector's: keep v0 and alpha_0 (= <v0|H v0>, FP64).
+__global__ void lz_guard(LzState* s, int refine) {
+    if (s->err) return;
+    const double e = refine >= 2 ? s->scal[6] : s->scal[3];
+    if (e > s->alpha[0]) {
+        s->kept_start = 1;
+        s->scal[3] = s->alpha[0];
+        s->scal[6] = s->alpha[0];
+    }
+}
+
+// psi = v0 when kept_start; partial sums of |H v0 - alpha_0 v0|^2 (its residual)
+__global__ __launch_bounds__(256) void lz_restore_start(const double* __restrict__ v0, const double* __restrict__ hv0,
+                                                        double* __restrict__ psi, int64_t n, const LzState* s,
+                                                        double* __restrict__ part) {
+    if (s->err || !s->kept_start) return;
+    const double a = s->alpha[0];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        psi[i] = v0[i];
+        const double r = hv0[i] - a * v0[i];
+        acc += r * r;
+    }
+    acc = block_sum256(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void lz_reduce_kept(const double* __restrict__ part, LzState* s) {
+    if (s->err || !s->kept_start) return;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) acc += part[i];
+    acc = block_sum256(acc);
+    if (threadIdx.x == 0) s->scal[4] = acc;
+}
+
 inline int grid_n(int64_t n) {
     int64_t g = (n + 255) / 256;
     return (int)(g < 1 ? 1 : (g > 4096 ? 4096 : g));
@@ -505,28 +578,39 @@ int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double*
     CT_ARG(max_iter >= 1 && max_iter <= 64, 5, "max_iter must be in [1, 64]");
     void *kb, *wb, *pb, *sb;
     CT_CHECK(ws_get(h, WS_KRYLOV, (size_t)dim * max_iter * 8, &kb));
-    CT_CHECK(ws_get(h, WS_W, (size_t)dim * 2 * 8, &wb));
+    CT_CHECK(ws_get(h, WS_W, (size_t)dim * 3 * 8, &wb));
     CT_CHECK(ws_get(h, WS_PARTIAL, NB * 8, &pb));
     CT_CHECK(ws_get(h, WS_LZ, sizeof(LzState), &sb));
     double* K = (double*)kb;
     double* w = (double*)wb;
     double* w2 = w + dim;
+    double* hv0 = w + 2 * dim;  // FP64 H v0, kept for the refinement (exact_start)
     double* part = (double*)pb;
     LzState* s = (LzState*)sb;
     const int* act = &s->active;
     const double thr = 100.0 * (double)dim * 2.2204460492503131e-16;
     const double floor_rel = env.prec == CT_PREC_FP64 ? 0.0 : 1e-5;
+    const bool exact_start = env.prec != CT_PREC_FP64 && refine >= 1 && env64 != nullptr;
     hipStream_t st = h->stream;
     const int G = grid_n(dim);
+    // Krylov matvec of v_j: the first one in FP64 when exact_start (and saved)
+    auto krylov_matvec = [&](int j, const double* vj) -> int {
+        if (j == 0 && exact_start) {
+            CT_CHECK(heff_apply(h, *env64, CT_F64, vj, w, act));
+            hipLaunchKernelGGL(lz_copy, dim3(G), dim3(256), 0, st, (const double*)w, hv0, dim, (const LzState*)s);
+            return 0;
+        }
+        return heff_apply(h, env, CT_F64, vj, w, act);
+    };
 
-    hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(1), 0, st, s, max_iter);
+    hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(1), 0, st, s, max_iter, exact_start ? 1 : 0);
     hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)d_theta, (const double*)nullptr, dim,
                        part, (const LzState*)s, 0);
     hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 0, thr, 0);
     hipLaunchKernelGGL(lz_start, dim3(G), dim3(256), 0, st, (const double*)d_theta, K, dim, s);
     for (int j = 0; j < max_iter - 1; ++j) {
         double* vj = K + (int64_t)j * dim;
-        CT_CHECK(heff_apply(h, env, CT_F64, vj, w, act));
+        CT_CHECK(krylov_matvec(j, vj));
         hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)vj, (const double*)w, dim, part,
                            (const LzState*)s, 1);
         hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, j, thr, 1);
@@ -537,7 +621,7 @@ int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double*
     }
     {   // final alpha-only step when no breakdown happened (dmrg.mojo:691-703)
         double* vj = K + (int64_t)(max_iter - 1) * dim;
-        CT_CHECK(heff_apply(h, env, CT_F64, vj, w, act));
+        CT_CHECK(krylov_matvec(max_iter - 1, vj));
         hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)vj, (const double*)w, dim, part,
                            (const LzState*)s, 1);
         hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, max_iter - 1, thr, 1);
@@ -552,8 +636,12 @@ int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double*
     CT_HIP(hipGetLastError());
 
     if (refine >= 1 && env64) {
-        // E_rq = <psi|H psi> with FP64 environments
-        CT_CHECK(heff_apply(h, *env64, CT_F64, d_theta, w, nullptr));
+        // E_rq = <psi|H psi> with FP64 environments (psi = v0 and the saved
+        // H v0 when the Krylov space has one vector)
+        if (exact_start)
+            hipLaunchKernelGGL(lz_take_start, dim3(G), dim3(256), 0, st, (const double*)K, (const double*)hv0, d_theta,
+                               w, dim, s);
+        CT_CHECK(heff_apply(h, *env64, CT_F64, d_theta, w, exact_start ? &s->refmv : nullptr));
         hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)d_theta, (const double*)w, dim,
                            part, (const LzState*)s, 0);
         hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 3, thr, 0);
@@ -568,6 +656,12 @@ int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double*
                                part, (const LzState*)s, 0);
             hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 5, thr, 0);
             hipLaunchKernelGGL(lz_correct, dim3(G), dim3(256), 0, st, d_theta, (const double*)w2, dim, s);
+        }
+        if (exact_start) {
+            hipLaunchKernelGGL(lz_guard, dim3(1), dim3(1), 0, st, s, refine);
+            hipLaunchKernelGGL(lz_restore_start, dim3(NB), dim3(256), 0, st, (const double*)K, (const double*)hv0,
+                               d_theta, dim, (const LzState*)s, part);
+            hipLaunchKernelGGL(lz_reduce_kept, dim3(1), dim3(256), 0, st, (const double*)part, s);
         }
         CT_HIP(hipGetLastError());
     }
@@ -594,7 +688,12 @@ int lanczos_finish(Handle* h, int refine, bool have_env64, LanczosResult* res) {
     res->E_rq = hs->scal[3];
     res->res_norm = sqrt(hs->scal[4]);
     res->E0 = res->E_lanczos;
-    if (refine >= 1 && have_env64) res->E0 = refine >= 2 ? hs->scal[6] : hs->scal[3];
+    res->kept_start = hs->kept_start;
+    res->matvecs = hs->numiter;
+    if (refine >= 1 && have_env64) {
+        res->E0 = refine >= 2 ? hs->scal[6] : hs->scal[3];
+        res->matvecs += (hs->refmv ? 1 : 0) + (refine >= 2 ? 1 : 0);
+    }
     return 0;
 }
 

@@ -113,6 +113,8 @@ class DenseTensor:
     def reshape(self, shape: Sequence[int]) -> "DenseTensor":
         """View with a new shape (dense_tensor.mojo:614-635); dims must be >= 1."""
         shape = [int(s) for s in shape]
+        if len(shape) == 0:
+            raise ValueError("reshape requires rank >= 1")
         if any(s < 1 for s in shape):
             raise ValueError("reshape dimensions must be >= 1")
         if int(np.prod(shape)) != self.size:

@@ -36,10 +36,16 @@ class DMRGParams:
                     the Jacobi)
     svd_mode        SVD_GRAM (default: FP64 eigendecomposition of theta theta^T or
                     theta^T theta, 10-25x faster than a full SVD at D >= 512) or
-                    SVD_GESVD (full FP64 SVD, the reference's factorisation).  Both
-                    give the same kept subspace and discarded weight; the Gram
-                    form resolves singular vectors below ~1e-8 sigma_max less
-                    accurately, which moves energies by far less than 1e-10.
+                    SVD_GESVD (full FP64 SVD, the reference's factorisation).  The
+                    Gram eigenvalues sigma^2 carry an absolute error of about
+                    k * 1e-16 * sigma_max^2 (k = min(Dl d, d Dr)), so singular
+                    values below ~sqrt(k) * 1e-8 sigma_max are resolved only
+                    roughly.  For eps_trunc well above k * 1e-16 both modes keep
+                    the same chi and discarded weight; for smaller eps_trunc (and
+                    chi_max not binding) the Gram split can keep a few more or
+                    fewer vectors than the full SVD, a difference of order
+                    k * 1e-16 in the discarded weight; use SVD_GESVD when that
+                    matters.
     graphs          replay each bond's Lanczos solve from a HIP graph cached per
                     bond once its shapes repeat (identical results; removes the
                     per-kernel launch cost that dominates at small bond dimensions)
@@ -77,6 +83,7 @@ class DMRGStats:
     matvec_flops: float = 0.0
     env_flops: float = 0.0
     graph_replays: int = 0
+    kept_start: int = 0
     sweep_energies: list = field(default_factory=list)
 
 
@@ -164,16 +171,24 @@ class HeffOperator:
             pass
 
 
+def _common_dtype(*tensors: DenseTensor) -> torch.dtype:
+    dt = tensors[0].dtype
+    for t in tensors[1:]:
+        if t.dtype != dt:
+            raise TypeError("all tensors must share one dtype (float64 or float32)")
+    return dt
+
+
 def lanczos_two_site_optimize(initial_theta: DenseTensor, L_env: DenseTensor, R_env: DenseTensor, W_i: MPOSite,
                               W_ip1: MPOSite, ctx: DeviceContext, max_iter: int, tol: float,
                               prec: int = PREC_FP64, refine: int = 0, return_info: bool = False,
                               max_rotations: int = 200):
     """Lowest eigenpair of the local H_eff (dmrg.mojo:486-756), FP64 Krylov basis on
-    the GPU.  Returns (E0, theta*) like the reference; with return_info also a dict
-    with the Krylov dimension, the raw Lanczos value and the residual norm."""
-    for t in (initial_theta, L_env, R_env, W_i.tensor, W_ip1.tensor):
-        if t.dtype != torch.float64:
-            raise TypeError("lanczos_two_site_optimize works on FP64 tensors (prec selects the matvec precision)")
+    the GPU.  Returns (E0, theta*) like the reference, theta* in the tensors' dtype
+    (float64, or the reference's float32, widened to FP64 inside the solver); with
+    return_info also a dict with the Krylov dimension, the raw Lanczos value and
+    the residual norm.  prec selects the matvec precision."""
+    dt = _common_dtype(initial_theta, L_env, R_env, W_i.tensor, W_ip1.tensor)
     Dl, d, _, Dr = initial_theta.shape
     wL, wM, wR = L_env.shape[0], W_i.tensor.shape[3], R_env.shape[0]
     th = initial_theta.storage.clone()
@@ -181,55 +196,61 @@ def lanczos_two_site_optimize(initial_theta: DenseTensor, L_env: DenseTensor, R_
     n = ctypes.c_int()
     El = ctypes.c_double()
     rn = ctypes.c_double()
-    call("ct_lanczos_two_site", ctx.handle, int(prec), L_env.ptr(), W_i.tensor.ptr(), W_ip1.tensor.ptr(),
-         R_env.ptr(), ctypes.c_void_p(th.data_ptr()), Dl, d, Dr, wL, wM, wR, int(max_iter), float(tol),
-         int(max_rotations), int(refine), ctypes.byref(E), ctypes.byref(n), ctypes.byref(El), ctypes.byref(rn))
+    call("ct_lanczos_two_site", ctx.handle, int(prec), initial_theta.dtype_code, L_env.ptr(), W_i.tensor.ptr(),
+         W_ip1.tensor.ptr(), R_env.ptr(), ctypes.c_void_p(th.data_ptr()), Dl, d, Dr, wL, wM, wR, int(max_iter),
+         float(tol), int(max_rotations), int(refine), ctypes.byref(E), ctypes.byref(n), ctypes.byref(El),
+         ctypes.byref(rn))
     out = DenseTensor(th, [Dl, d, d, Dr])
     if return_info:
         return E.value, out, {"num_iter": n.value, "E_lanczos": El.value, "residual_norm": rn.value}
     return E.value, out
 
 
-def dense_tensor_svd_trunc(tensor: DenseTensor, ctx: DeviceContext, chi_max: int, eps_trunc: float = 1e-12):
-    """Truncated SVD of a [m, n] FP64 matrix on the GPU (dense_tensor.mojo:1483-1623).
-    Returns (U [m,chi], S [chi], Vt [chi,n], chi, discarded_weight)."""
+def dense_tensor_svd_trunc(tensor: DenseTensor, ctx: DeviceContext, chi_max: int, eps_trunc: float = 1e-12,
+                           return_discarded: bool = False):
+    """Truncated SVD of a [m, n] matrix on the GPU (dense_tensor.mojo:1483-1623), in FP64
+    (float32 input is widened and the factors rounded back).  Returns (U [m,chi],
+    S [chi], Vt [chi,n], chi) like the reference; with return_discarded also the
+    discarded weight."""
     if tensor.rank() != 2:
         raise ValueError("dense_tensor_svd_trunc expects a matrix")
-    if tensor.dtype != torch.float64:
-        raise TypeError("dense_tensor_svd_trunc works in FP64")
+    dt = tensor.dtype
     m, n = tensor.shape
     k = min(m, n, chi_max)
     dev = tensor.storage.device
-    U = torch.empty(max(m * k, 1), dtype=torch.float64, device=dev)
-    S = torch.empty(max(k, 1), dtype=torch.float64, device=dev)
-    Vt = torch.empty(max(k * n, 1), dtype=torch.float64, device=dev)
+    U = torch.empty(max(m * k, 1), dtype=dt, device=dev)
+    S = torch.empty(max(k, 1), dtype=dt, device=dev)
+    Vt = torch.empty(max(k * n, 1), dtype=dt, device=dev)
     chi = ctypes.c_int64()
     dw = ctypes.c_double()
-    call("ct_svd_trunc", ctx.handle, tensor.ptr(), m, n, chi_max, float(eps_trunc), ctypes.c_void_p(U.data_ptr()),
-         ctypes.c_void_p(S.data_ptr()), ctypes.c_void_p(Vt.data_ptr()), ctypes.byref(chi), ctypes.byref(dw))
+    call("ct_svd_trunc", ctx.handle, tensor.dtype_code, tensor.ptr(), m, n, chi_max, float(eps_trunc),
+         ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(S.data_ptr()), ctypes.c_void_p(Vt.data_ptr()),
+         ctypes.byref(chi), ctypes.byref(dw))
     c = chi.value
     if c == 0:
-        return None, None, None, 0, dw.value
-    return (DenseTensor(U[: m * c], [m, c]), DenseTensor(S[:c], [c]), DenseTensor(Vt[: c * n], [c, n]), c,
-            dw.value)
+        res = (None, None, None, 0)
+    else:
+        res = (DenseTensor(U[: m * c], [m, c]), DenseTensor(S[:c], [c]), DenseTensor(Vt[: c * n], [c, n]), c)
+    return res + (dw.value,) if return_discarded else res
 
 
 def split_two_site_theta_svd(theta: DenseTensor, chi_max: int, eps_trunc: float, ctx: DeviceContext,
                              left_to_right: bool = True, svd_mode: int = SVD_GESVD):
     """theta[Dl,d,d,Dr] -> (A_i[Dl,d,chi], A_{i+1}[chi,d,Dr], discarded_weight)
-    (dmrg.mojo:143-252): S goes right on L->R, left on R->L.  FP64 on the GPU."""
-    if theta.dtype != torch.float64:
-        raise TypeError("split_two_site_theta_svd works in FP64")
+    (dmrg.mojo:143-252): S goes right on L->R, left on R->L.  The split runs in
+    FP64 on the GPU; float32 theta (the reference's dtype) is widened on entry and
+    the site tensors are returned in float32."""
+    dt = theta.dtype
     Dl, d, _, Dr = theta.shape
     k = min(Dl * d, d * Dr, chi_max)
     dev = theta.storage.device
-    Ai = torch.empty(Dl * d * k, dtype=torch.float64, device=dev)
-    Aj = torch.empty(k * d * Dr, dtype=torch.float64, device=dev)
+    Ai = torch.empty(Dl * d * k, dtype=dt, device=dev)
+    Aj = torch.empty(k * d * Dr, dtype=dt, device=dev)
     chi = ctypes.c_int64()
     dw = ctypes.c_double()
-    call("ct_theta_split", ctx.handle, int(svd_mode), theta.ptr(), Dl, d, Dr, chi_max, float(eps_trunc),
-         int(bool(left_to_right)), ctypes.c_void_p(Ai.data_ptr()), ctypes.c_void_p(Aj.data_ptr()), ctypes.byref(chi),
-         ctypes.byref(dw))
+    call("ct_theta_split", ctx.handle, int(svd_mode), theta.dtype_code, theta.ptr(), Dl, d, Dr, chi_max,
+         float(eps_trunc), int(bool(left_to_right)), ctypes.c_void_p(Ai.data_ptr()), ctypes.c_void_p(Aj.data_ptr()),
+         ctypes.byref(chi), ctypes.byref(dw))
     c = chi.value
     if c == 0:
         raise RuntimeError("truncation kept no singular value")
@@ -252,9 +273,12 @@ class DMRGRun:
         self.N = N
         self.d = mps.physical_dim
         self.params = params
-        # FP64 device copies (the driver keeps FP64 tensors; prec selects the matvec precision)
-        self._mpo = [s.tensor.storage.to(torch.float64) for s in mpo.sites]
-        self._mps = [s.tensor.storage.to(torch.float64) for s in mps.sites]
+        # The driver copies the sites into its own FP64 storage at create time;
+        # float32 sites (the reference's dtype) are widened there and mps()
+        # returns float32 again.
+        self.dtype = _common_dtype(*[s.tensor for s in mps.sites], *[s.tensor for s in mpo.sites])
+        self._mpo = [s.tensor.storage.contiguous() for s in mpo.sites]
+        self._mps = [s.tensor.storage.contiguous() for s in mps.sites]
         p = _lib.DMRGParamsC(params.num_sweeps, params.chi_max, params.eps_trunc, params.max_krylov_iter,
                              params.krylov_tol, params.energy_tol, params.jacobi_max_rot, params.svd_mode, params.prec,
                              params.refine, params.env_prec, int(params.verbose), int(params.graphs))
@@ -263,16 +287,17 @@ class DMRGRun:
         wp = (ctypes.c_void_p * N)(*[t.data_ptr() for t in self._mpo])
         mp = (ctypes.c_void_p * N)(*[t.data_ptr() for t in self._mps])
         st = ctypes.c_void_p()
-        call("ct_dmrg_create", ctx.handle, ctypes.byref(p), N, self.d, wd, wp, md, mp, ctypes.byref(st))
+        call("ct_dmrg_create", ctx.handle, ctypes.byref(p), N, self.d, _lib.F64 if self.dtype == torch.float64
+             else _lib.F32, wd, wp, md, mp, ctypes.byref(st))
         self.state = st
 
     def run(self, num_sweeps: int = 0) -> tuple[float, list[float]]:
         n = num_sweeps if num_sweeps > 0 else self.params.num_sweeps
         E = ctypes.c_double()
         se = (ctypes.c_double * max(n, 1))()
-        call("ct_dmrg_run", self.state, n, ctypes.byref(E), se)
-        st = self.stats()
-        return E.value, list(se)[: max(0, min(n, st.sweeps))]
+        ran = ctypes.c_int()
+        call("ct_dmrg_run", self.state, n, ctypes.byref(E), se, ctypes.byref(ran))
+        return E.value, list(se)[: ran.value]
 
     def stats(self) -> DMRGStats:
         c = _lib.DMRGStatsC()
@@ -289,8 +314,9 @@ class DMRGRun:
         sites = []
         for i in range(self.N):
             n = dims[i] * self.d * dims[i + 1]
-            t = torch.empty(n, dtype=torch.float64, device=self.ctx.device)
-            call("ct_dmrg_get_site", self.state, i, ctypes.c_void_p(t.data_ptr()))
+            t = torch.empty(n, dtype=self.dtype, device=self.ctx.device)
+            call("ct_dmrg_get_site", self.state, i, _lib.F64 if self.dtype == torch.float64 else _lib.F32,
+                 ctypes.c_void_p(t.data_ptr()))
             sites.append(MPSSite(DenseTensor(t, [dims[i], self.d, dims[i + 1]])))
         return MatrixProductState(sites)
 

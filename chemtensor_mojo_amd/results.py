@@ -154,8 +154,7 @@ def bond_entropies(mps: MatrixProductState, ctx: DeviceContext) -> list[float]:
     N = len(sites)
     for i in range(N - 1, 0, -1):  # right-canonical sites 1..N-1, centre on site 0
         Dl, d, Dr = sites[i].shape
-        U, S, Vt, c, _ = dense_tensor_svd_trunc(DenseTensor(sites[i].storage, [Dl, d * Dr]), ctx, min(Dl, d * Dr),
-                                                0.0)
+        U, S, Vt, c = dense_tensor_svd_trunc(DenseTensor(sites[i].storage, [Dl, d * Dr]), ctx, min(Dl, d * Dr), 0.0)
         sites[i] = DenseTensor(Vt.storage, [c, d, Dr])
         US = DenseTensor((U.storage.view(Dl, c) * S.storage.view(1, c)).reshape(-1), [Dl, c])
         Dp, dp, _ = sites[i - 1].shape
@@ -165,8 +164,7 @@ def bond_entropies(mps: MatrixProductState, ctx: DeviceContext) -> list[float]:
     ent = []
     for i in range(N - 1):  # move the centre right, recording each bond's spectrum
         Dl, d, Dr = sites[i].shape
-        U, S, Vt, c, _ = dense_tensor_svd_trunc(DenseTensor(sites[i].storage, [Dl * d, Dr]), ctx, min(Dl * d, Dr),
-                                                0.0)
+        U, S, Vt, c = dense_tensor_svd_trunc(DenseTensor(sites[i].storage, [Dl * d, Dr]), ctx, min(Dl * d, Dr), 0.0)
         s2 = S.storage.double() ** 2
         p = (s2 / s2.sum()).cpu().numpy()
         p = p[p > 0]

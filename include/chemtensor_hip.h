@@ -177,16 +177,22 @@ int ct_two_site_theta(ct_handle_t h, int prec, int dtype, const void* A_i, const
  * at most max_rotations rotations - the reference's cap is 200, dmrg.mojo:510 -;
  * max_rotations = -1 instead solves T exactly (Sturm multisection + inverse
  * iteration, the converged Jacobi result to rounding),
- * normalised FP64 Ritz vector).  theta (FP64, [Dl,d,d,Dr]) is the start
- * vector on entry and the Ritz vector on return.
- * L, R, Wi, Wj are FP64.  prec selects the matvec precision; the FP32 and
+ * normalised FP64 Ritz vector).  theta ([Dl,d,d,Dr]) is the start
+ * vector on entry and the Ritz vector on return.  dtype is the element type
+ * of L, R, Wi, Wj and theta: CT_F64, or CT_F32 (the reference's float32
+ * tensors, dmrg.mojo:486-495), which are widened to FP64 on entry and theta
+ * rounded back on return; the Krylov basis, the tridiagonal solve and the
+ * refinement stay FP64 either way.  prec selects the matvec precision; the FP32 and
  * 3xBF16 forms also stop once beta < 1e-5 times max |alpha|, the noise
  * level of their matvecs.  refine != 0 adds
  * an FP64 Rayleigh quotient of the Ritz vector (E_rq) and, refine == 2, one
  * residual-correction step in span{psi, H psi - E_rq psi}; E0 then returns the
- * refined energy.  Synchronises the stream.  num_iter = Krylov dimension. */
-int ct_lanczos_two_site(ct_handle_t h, int prec, const double* L, const double* Wi, const double* Wj,
-                        const double* R, double* theta, int64_t Dl, int64_t d, int64_t Dr, int64_t wL,
+ * refined energy.  With refinement of a mixed-precision solve, the first
+ * Krylov matvec runs in FP64 and the start vector (energy alpha_0) is
+ * returned whenever the refined energy would lie above it.  Synchronises
+ * the stream.  num_iter = Krylov dimension. */
+int ct_lanczos_two_site(ct_handle_t h, int prec, int dtype, const void* L, const void* Wi, const void* Wj,
+                        const void* R, void* theta, int64_t Dl, int64_t d, int64_t Dr, int64_t wL,
                         int64_t wM, int64_t wR, int max_iter, double tol, int max_rotations, int refine,
                         double* E0, int* num_iter, double* E_lanczos, double* residual_norm);
 
@@ -196,17 +202,22 @@ int ct_lanczos_two_site(ct_handle_t h, int prec, const double* L, const double* 
  * (sum s^2 - sum_{<chi} s^2)/sum s^2 <= eps, else min(m,n,chi_max)
  * (:1528-1548).  U/S/Vt must hold min(m,n,chi_max) columns/values/rows.
  * Also returns the discarded weight the reference computes and drops
- * (dmrg.mojo:188-200, :824).  Synchronises the stream. */
-int ct_svd_trunc(ct_handle_t h, const double* A, int64_t m, int64_t n, int64_t chi_max, double eps,
-                 double* U, double* S, double* Vt, int64_t* chi, double* discarded_weight);
+ * (dmrg.mojo:188-200, :824).  dtype (CT_F64 / CT_F32) is the element type
+ * of A, U, S and Vt; float32 operands (the reference's SVD input,
+ * dense_tensor.mojo:1500-1502) are factorised in FP64 and the factors
+ * rounded back.  Synchronises the stream. */
+int ct_svd_trunc(ct_handle_t h, int dtype, const void* A, int64_t m, int64_t n, int64_t chi_max, double eps,
+                 void* U, void* S, void* Vt, int64_t* chi, double* discarded_weight);
 
 /* split_two_site_theta_svd (src/algorithms/dmrg.mojo:143-252): theta
- * [Dl,d,d,Dr] (FP64, preserved) -> A_i [Dl,d,chi], A_j [chi,d,Dr] with S
+ * [Dl,d,d,Dr] (preserved) -> A_i [Dl,d,chi], A_j [chi,d,Dr] with S
  * absorbed into A_j (left_to_right) or into A_i (right-to-left).  A_i / A_j
  * must hold min(Dl*d, d*Dr, chi_max) bond columns / rows.  svd_mode selects
- * CT_SVD_GESVD or CT_SVD_GRAM.  Synchronises the stream. */
-int ct_theta_split(ct_handle_t h, int svd_mode, const double* theta, int64_t Dl, int64_t d, int64_t Dr,
-                   int64_t chi_max, double eps, int left_to_right, double* A_i, double* A_j, int64_t* chi,
+ * CT_SVD_GESVD or CT_SVD_GRAM.  dtype (CT_F64 / CT_F32) is the element type
+ * of theta, A_i and A_j; the split itself runs in FP64.  Synchronises the
+ * stream. */
+int ct_theta_split(ct_handle_t h, int svd_mode, int dtype, const void* theta, int64_t Dl, int64_t d, int64_t Dr,
+                   int64_t chi_max, double eps, int left_to_right, void* A_i, void* A_j, int64_t* chi,
                    double* discarded_weight);
 
 /* Symmetric eigensolver of the CT_SVD_GRAM split (no reference counterpart:
@@ -267,22 +278,29 @@ typedef struct {
     double matvec_flops;       /* algorithmic FLOPs of those matvecs (reference order) */
     double env_flops;
     int64_t graph_replays;     /* local solves replayed from a cached graph (ct_dmrg_params.graphs) */
+    int64_t kept_start;        /* refined mixed-precision solves whose result lay above the start vector's
+                                  FP64 energy, so the start vector was kept (no reference counterpart) */
 } ct_dmrg_stats;
 
-/* mpo_dims[nsites+1], mpo_sites[i]: device FP64 [w_i,d,d,w_{i+1}];
- * mps_dims[nsites+1], mps_sites[i]: device FP64 [D_i,d,D_{i+1}]. */
-int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int64_t d, const int64_t* mpo_dims,
-                   const double* const* mpo_sites, const int64_t* mps_dims, const double* const* mps_sites,
-                   ct_dmrg_t* out);
+/* mpo_dims[nsites+1], mpo_sites[i]: device [w_i,d,d,w_{i+1}];
+ * mps_dims[nsites+1], mps_sites[i]: device [D_i,d,D_{i+1}];
+ * dtype (CT_F64 / CT_F32) is the element type of the given sites (float32 is
+ * the reference's storage, dmrg.mojo:377-382).  The driver keeps FP64 copies;
+ * params.prec selects the matvec precision. */
+int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int64_t d, int dtype,
+                   const int64_t* mpo_dims, const void* const* mpo_sites, const int64_t* mps_dims,
+                   const void* const* mps_sites, ct_dmrg_t* out);
 int ct_dmrg_destroy(ct_dmrg_t s);
 /* Runs params.num_sweeps (or num_sweeps if > 0) sweeps; sweep_energies (host,
- * may be NULL) receives one energy per sweep.  Synchronises. */
-int ct_dmrg_run(ct_dmrg_t s, int num_sweeps, double* energy, double* sweep_energies);
+ * may be NULL) receives one energy per sweep run, and sweeps_run (may be
+ * NULL) the number of sweeps this call ran (fewer than asked when
+ * params.energy_tol stops it, dmrg.mojo:466-474).  Synchronises. */
+int ct_dmrg_run(ct_dmrg_t s, int num_sweeps, double* energy, double* sweep_energies, int* sweeps_run);
 int ct_dmrg_get_stats(ct_dmrg_t s, ct_dmrg_stats* stats);
 /* current bond dimensions (nsites+1 values) */
 int ct_dmrg_get_dims(ct_dmrg_t s, int64_t* dims);
-/* copies site i ([D_i,d,D_{i+1}] FP64) into a device buffer */
-int ct_dmrg_get_site(ct_dmrg_t s, int i, double* dst);
+/* copies site i ([D_i,d,D_{i+1}]) into a device buffer of element type dtype */
+int ct_dmrg_get_site(ct_dmrg_t s, int i, int dtype, void* dst);
 
 /* ---- adjoint (no reference counterpart: new capability) -----------------
  * dE/dW_i of E = <psi|H|psi> for a two-site wave function theta at sites

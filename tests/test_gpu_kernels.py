@@ -452,7 +452,7 @@ def test_lanczos_refined_vector(ctx, ora, prec, refine):
 def test_svd_kat(ctx):
     """sigma = [3, 2] (test_svd_lapack.mojo:55-91)."""
     A = np.array([[2.0, 0, 0], [0, 3.0, 0], [0, 0, 0], [0, 0, 0]])
-    U, S, Vt, chi, dw = ct.dense_tensor_svd_trunc(dev(ctx, A), ctx, chi_max=2, eps_trunc=1e-12)
+    U, S, Vt, chi, dw = ct.dense_tensor_svd_trunc(dev(ctx, A), ctx, chi_max=2, eps_trunc=1e-12, return_discarded=True)
     assert chi == 2 and U.shape == [4, 2] and Vt.shape == [2, 3]
     assert np.allclose(S.to_numpy(), [3.0, 2.0], atol=1e-14)
 
@@ -464,7 +464,7 @@ def test_svd_trunc_vs_oracle(ctx, ora, mn, chi_max, eps):
     rng = np.random.default_rng(m + n)
     A = rng.standard_normal((m, n)) @ np.diag(np.exp(-0.1 * np.arange(n)))
     U_o, S_o, Vt_o, chi_o = ora.svd_trunc(A, chi_max, eps)
-    U, S, Vt, chi, dw = ct.dense_tensor_svd_trunc(dev(ctx, A), ctx, chi_max, eps)
+    U, S, Vt, chi, dw = ct.dense_tensor_svd_trunc(dev(ctx, A), ctx, chi_max, eps, return_discarded=True)
     assert chi == chi_o
     assert rel(S.to_numpy(), S_o) < 1e-12
     Ug, Vg = U.to_numpy(), Vt.to_numpy()
@@ -624,20 +624,25 @@ def test_dmrg_quantum_chemistry_number_sector_start(ctx, prec, refine):
     assert leak < 1e-10, leak
 
 
-@pytest.mark.parametrize("prec,refine", [(ct.PREC_FP64, 0), (ct.PREC_BF16X3, 2)])
-def test_dmrg_graph_replay_matches_eager(ctx, prec, refine):
+@pytest.mark.parametrize("prec,refine,max_rot", [(ct.PREC_FP64, 0, -1), (ct.PREC_BF16X3, 2, -1),
+                                                 (ct.PREC_FP32, 0, -1), (ct.PREC_FP32, 2, -1),
+                                                 (ct.PREC_FP64, 1, -1), (ct.PREC_FP64, 0, 200)])
+def test_dmrg_graph_replay_matches_eager(ctx, prec, refine, max_rot):
     """Each bond's Lanczos solve replayed from its cached HIP graph gives the same
     sweeps, bit for bit, as launching every kernel.  The bond dimension is fixed
     (eps_trunc = 0), so a bond's key repeats on every visit: the first visit runs
     eagerly, the second captures, every later one replays, with the eager
-    truncations and environment updates of the other bonds in between."""
+    truncations and environment updates of the other bonds in between.  Covers
+    every matvec precision (FP32 captures its own operand conversions),
+    refinement 0/1/2, and the reference's 200-rotation Jacobi cap besides the
+    exact tridiagonal solve."""
     L = 12
     mpo_t = ham.j1j2_tensors(L, 1.0, 0.5)
     mps_t = ct.random_mps_c_compatible_tensors(L, 2, 16, 7)
     runs = {}
     for graphs in (False, True):
         p = ct.DMRGParams(num_sweeps=4, chi_max=16, eps_trunc=0.0, max_krylov_iter=8, krylov_tol=1e-12,
-                          energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=-1, graphs=graphs)
+                          energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=max_rot, graphs=graphs)
         E, out, st = ct.dmrg_two_site(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p,
                                       return_stats=True)
         runs[graphs] = (E, [s.tensor.to_numpy() for s in out.sites], st)
