@@ -5,16 +5,20 @@ dimension D = 1024 - the precision configuration of the reference's
 benchmark set): one "step" is one two-site H_eff matvec
 (apply_two_site_heff, reference src/algorithms/dmrg.mojo:254-374) on
 HBM-resident L, R, W and theta.  The matvec is timed in FP64 (the
-oracle-equivalent path), FP32 and 3xBF16; ``value`` is the 3xBF16 matvec rate
-in algorithmic TFLOP/s (reference contraction order, 139.59 GFLOP per matvec)
-of the prepared operator the Lanczos uses (W_i W_{i+1} and the L/R copies built
-once per local problem); the one-shot rate, which rebuilds them every call, is
-reported beside it.
+oracle-equivalent path), FP32 and 3xBF16; ``value`` is the FP32 matvec rate
+(the reference computes in float32) in algorithmic TFLOP/s (reference
+contraction order, 139.59 GFLOP per matvec) of the prepared operator the
+Lanczos uses (W_i W_{i+1} and the L/R copies built once per local problem);
+the one-shot rate, which rebuilds them every call, is reported beside it.
 
-Also reported: a fixed-D DMRG sweep (FP64 and 3xBF16 + FP64 refinement) with
-its energy difference, the roofline of the dominant kernel (the last GEMM of
-the matvec, U[(Dl d d)][(wR Dr)] R[(wR Dr)][Dr']) timed live with events on
-the stream it runs on, and the FP64 CPU oracle timed on the host cores.
+Also reported: the roofline of the dominant kernel of each form (the last
+GEMM of the matvec, U[(Dl d d)][(wR Dr)] R[(wR Dr)][Dr']) timed live with HIP
+events on the stream it runs on; fixed-D D=1024 DMRG sweeps in FP64 and in
+the refined FP32 / 3xBF16 forms on two chains - one whose local problems are
+converged after the first sweep (Majumdar-Ghosh point) and one whose Lanczos
+runs its whole Krylov space (antiferromagnetic Heisenberg); the symmetric
+eigensolver of the truncation; the matvec grid over D and w; and the FP64 CPU
+oracle timed on the host cores.
 
 Prints one JSON line.  N > 1 GPUs is not supported (the path runs on one GPU).
 """
@@ -35,10 +39,6 @@ sys.path.insert(0, ROOT)
 METRIC = "DMRG sweep s + H_eff matvec TFLOP/s at D=1024; |ΔE/E| vs FP64 oracle"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # TFLOP/s, MI355X dense MFMA
 HBM_PEAK = 8000.0  # GB/s
-# HBM bytes of one call of the dominant GEMM at D=1024, w=8 (GEMM: 294.9 MB read +
-# 67.3 MB written; split-K reduction: 67.1 MB + 33.6 MB), from the FETCH_SIZE and
-# WRITE_SIZE counter runs in profiles/heff_matvec_j1j2_D1024_hbm_bytes.md.
-DOMINANT_TRAFFIC_BYTES = 462.9e6
 
 
 def log(*a):
@@ -192,6 +192,12 @@ def bench_dominant_gemm(ctx, D, w, steps, warmup):
     tC = ct.create_dense_tensor(ctx, [M, N])
     t, _ = time_events(lambda: ct.dense_tensor_dot(tC, tA, tB, ctx), steps, warmup)
     out["fp64"] = {"s": t, "tflops": F / t / 1e12, "bytes": 8.0 * (M * K + K * N + M * N)}
+    # FP32 (the reference's precision): the same product on float32 copies
+    tA32 = ct.DenseTensor(A64.float(), [M, K])
+    tB32 = ct.DenseTensor(B64.float(), [K, N])
+    tC32 = ct.create_dense_tensor(ctx, [M, N], dtype=torch.float32)
+    t, _ = time_events(lambda: ct.dense_tensor_dot(tC32, tA32, tB32, ctx), steps, warmup)
+    out["fp32"] = {"s": t, "tflops": F / t / 1e12, "bytes": 4.0 * (M * K + K * N + M * N)}
     # 3xBF16: pre-split operands (split rows), time only the NT MFMA kernels
     ld = 2 * ((K + 31) // 32 * 32)
     As = torch.empty(M * ld, dtype=torch.int16, device="cuda")
@@ -207,6 +213,7 @@ def bench_dominant_gemm(ctx, D, w, steps, warmup):
     ref = (A64.view(M, K) @ B64.view(K, N)).reshape(-1)
     out["bf16x3"]["rel_err"] = float(torch.linalg.norm(C32 - ref) / torch.linalg.norm(ref))
     out["fp64"]["rel_err"] = float(torch.linalg.norm(tC.storage - ref) / torch.linalg.norm(ref))
+    out["fp32"]["rel_err"] = float(torch.linalg.norm(tC32.storage.double() - ref) / torch.linalg.norm(ref))
     for k, v in out.items():
         log(f"dominant GEMM {k}: {v['s'] * 1e3:.3f} ms {v['tflops']:.2f} TFLOP/s (err {v['rel_err']:.1e})")
     # Calibration stream for hardware byte counters: ct_scale in place over
@@ -216,18 +223,20 @@ def bench_dominant_gemm(ctx, D, w, steps, warmup):
         _lib.call("ct_scale", ctx.handle, 0, 1.0, vp(cal), cal.numel())
     torch.cuda.synchronize()
     del cal
-    del A64, B64, As, Bs, C32, tC
+    del A64, B64, As, Bs, C32, tC, tA32, tB32, tC32
     torch.cuda.empty_cache()
     return out, F
 
 
-def bench_sweep(ctx, L_sites, D, prec, refine, J2=0.5, seed=42, sweeps=2, max_krylov=20):
+def bench_sweep(ctx, L_sites, D, prec, refine, J2=0.5, seed=42, sweeps=2, max_krylov=20, model="j1j2"):
     """Fixed-D two-site sweeps (eps_trunc = 0, chi_max = D) of the J1-J2 chain
-    from a seeded PCG32 random MPS whose bonds already sit at min(2^l, D); the
-    last sweep's wall time and stage times are the steady-state figures."""
+    (model "j1j2") or the antiferromagnetic Heisenberg chain ("heisenberg",
+    +sum(XX+YY+ZZ), w=5) from a seeded PCG32 random MPS whose bonds already sit
+    at min(2^l, 2^(L-l), D); per-sweep wall and stage times."""
     import chemtensor_mojo_amd as ct
     from chemtensor_mojo_amd import hamiltonians as ham
-    mpo = ct.mpo_from_tensors(ctx, ham.j1j2_tensors(L_sites, 1.0, J2))
+    mpo_t = ham.j1j2_tensors(L_sites, 1.0, J2) if model == "j1j2" else ham.heisenberg_tensors(L_sites, 1.0)
+    mpo = ct.mpo_from_tensors(ctx, mpo_t)
     mps = ct.create_random_mps_c_compatible(ctx, L_sites, 2, D, seed)
     p = ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=0.0, max_krylov_iter=max_krylov, krylov_tol=1e-12,
                       energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=-1, svd_mode=ct.SVD_GRAM)
@@ -251,13 +260,12 @@ def bench_sweep(ctx, L_sites, D, prec, refine, J2=0.5, seed=42, sweeps=2, max_kr
         st = run.stats()
     finally:
         run.close()
-    last = stages[-1]
-    return {"energies": Es, "sweep_s": times,
-            "last_sweep": {"s": times[-1], "lanczos_s": last["t_lanczos"], "svd_s": last["t_svd"],
-                           "env_s": last["t_env"], "theta_s": last["t_theta"], "matvecs": int(last["matvecs"]),
-                           "matvec_tflops_in_sweep": (last["matvec_flops"] / last["t_lanczos"] / 1e12
-                                                      if last["t_lanczos"] > 0 else None)},
-            "max_bond": st.max_bond, "trunc_weight_sum": st.trunc_weight_sum}
+    per = [{"s": times[i], "lanczos_s": g["t_lanczos"], "svd_s": g["t_svd"], "env_s": g["t_env"],
+            "theta_s": g["t_theta"], "matvecs": int(g["matvecs"]),
+            "matvec_tflops_in_sweep": (g["matvec_flops"] / g["t_lanczos"] / 1e12 if g["t_lanczos"] > 0 else None)}
+           for i, g in enumerate(stages)]
+    return {"energies": Es, "sweep_s": times, "per_sweep": per, "last_sweep": per[-1],
+            "max_bond": st.max_bond, "trunc_weight_sum": st.trunc_weight_sum, "kept_start": st.kept_start}
 
 
 # ------------------------------------------------------------------ studies
@@ -633,7 +641,37 @@ def cpu_baseline(D, w, budget_s):
             break
     t = (time.perf_counter() - t0) / reps
     return {"value": F / t / 1e12, "unit": "TFLOP/s", "cores": ora.get_num_threads(), "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"{reps} FP64 H_eff matvecs at D={D}, w={w} (oracle, OpenBLAS dgemm, {t:.2f} s each)"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def profiled_traffic(key, D, w):
+    """HBM bytes per call of a dominant kernel at D=1024, w=8, from the counter
+    summary committed in profiles/ (hardware counters cannot be read from inside
+    this process); None when the summary has no row for the key."""
+    if (D, w) != (1024, 8):
+        return None
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_D1024_w8.md")
+    try:
+        with open(path) as f:
+            for ln in f:
+                cells = [c.strip() for c in ln.strip().strip("|").split("|")]
+                if len(cells) >= 5 and cells[0] == key:
+                    return float(cells[4])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -645,6 +683,7 @@ def main():
     ap.add_argument("--w", type=int, default=8)
     ap.add_argument("--sweep-sites", type=int, default=24)
     ap.add_argument("--sweep-D", type=int, default=1024)
+    ap.add_argument("--afm-sites", type=int, default=40)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the eigensolver and matvec-grid sections")
@@ -677,25 +716,37 @@ def main():
     mgrid = None if args.no_extras else bench_matvec_grid(ctx, max(5, args.steps // 2), args.warmup)
     sweep = None
     if not args.no_sweep:
-        s64 = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_FP64, 0)
-        smx = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_BF16X3, 2)
-        s32 = bench_sweep(ctx, args.sweep_sites, args.sweep_D, ct.PREC_FP32, 2)
-        E64 = s64["energies"][-1]
-        sweep = {"workload": (f"J1-J2 chain (J2=0.5), L={args.sweep_sites}, fixed D={args.sweep_D} (eps_trunc=0), "
-                              "PCG32 random start at full D, 20 Krylov vectors, Gram-matrix FP64 truncation, 2 sweeps"),
-                 "fp64": s64, "bf16x3_refined": smx, "fp32_refined": s32,
-                 "dE_rel_bf16x3_vs_fp64": abs(smx["energies"][-1] - E64) / abs(E64),
-                 "dE_rel_fp32_vs_fp64": abs(s32["energies"][-1] - E64) / abs(E64),
-                 "speedup_sweep_bf16x3": s64["sweep_s"][-1] / smx["sweep_s"][-1],
-                 "speedup_sweep_fp32": s64["sweep_s"][-1] / s32["sweep_s"][-1],
-                 "note": "FP64 GPU path is the oracle-equivalent reference here (parity-tested against the CPU "
-                         "oracle at small D); last-sweep stage times in last_sweep"}
+        sweep = {}
+        for key, model, L_sites, nsw in (("j1j2_mg", "j1j2", args.sweep_sites, 2),
+                                         ("heisenberg_afm", "heisenberg", args.afm_sites, 2)):
+            runs = {name: bench_sweep(ctx, L_sites, args.sweep_D, prec, refine, model=model, sweeps=nsw)
+                    for name, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
+                                               ("bf16x3_refined", ct.PREC_BF16X3, 2))}
+            E64 = runs["fp64"]["energies"][-1]
+            desc = ("J1-J2 chain at the Majumdar-Ghosh point (J2=0.5, w=8): after the first sweep every local "
+                    "problem is converged, so the sweep times the truncation and a one-vector Lanczos"
+                    if model == "j1j2" else
+                    "antiferromagnetic Heisenberg chain (w=5), far from converged at this bond dimension: every "
+                    "local Lanczos runs its full Krylov space")
+            sweep[key] = {
+                "workload": (f"{desc}; L={L_sites}, fixed D={args.sweep_D} (eps_trunc=0), PCG32 random start at "
+                             f"full D, 20 Krylov vectors, exact tridiagonal solves, Gram-matrix FP64 truncation, "
+                             f"{nsw} sweeps"),
+                **runs,
+                "dE_rel_bf16x3_vs_fp64": abs(runs["bf16x3_refined"]["energies"][-1] - E64) / abs(E64),
+                "dE_rel_fp32_vs_fp64": abs(runs["fp32_refined"]["energies"][-1] - E64) / abs(E64),
+                "speedup_sweep_bf16x3": [a / b for a, b in zip(runs["fp64"]["sweep_s"],
+                                                               runs["bf16x3_refined"]["sweep_s"])],
+                "speedup_sweep_fp32": [a / b for a, b in zip(runs["fp64"]["sweep_s"], runs["fp32_refined"]["sweep_s"])],
+            }
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(D, w, args.cpu_budget)
-        log(f"cpu baseline: {cpu['value']:.3f} TFLOP/s on {cpu['cores']} threads")
-    head = mv["bf16x3"]
-    dom = gm["bf16x3"]
+        log(f"cpu baseline: {cpu['value']:.3f} TFLOP/s on {cpu['cores']} threads ({cpu['cpu_model']})")
+    head = mv["fp32"]
+    dom = gm["fp32"]
+    dom_kernel = (f"gemm_mfma_kernel<float> + gemm_splitk_reduce<float> (stage-4 GEMM of the FP32 matvec, "
+                  f"[{4 * D}x{w * D}].[{w * D}x{D}], 128x128 tiles, split-K 2)")
     line = {
         "metric": METRIC,
         "value": round(head["tflops"], 3),
@@ -707,24 +758,32 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3 (fp32 accumulate)",
+        "dtype": "fp32",
         "data": "synthetic (seeded random environments/theta, J1-J2 bulk MPO)",
         "config": {"workload": f"two-site H_eff matvec, J1-J2 chain, D={D}, w={w}, d=2",
                    "matvec_gflop": round(F / 1e9, 3),
-                   "value_basis": "prepared H_eff operator (W_i W_i+1 and the L/R copies built once per local "
-                                  "problem, as inside the Lanczos); one-shot figures in matvec.*_one_shot"},
+                   "value_basis": "FP32 matvec (the reference's float32 arithmetic) of the prepared H_eff operator "
+                                  "(W_i W_i+1 and the L/R copies built once per local problem, as inside the "
+                                  "Lanczos); FP64 and 3xBF16 forms and the one-shot figures in matvec"},
         "matvec": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in mv.items()},
-        "roofline": {"bound": "mfma",
-                     "kernel": "gemm_bf16x3_dma_kernel<float,true> + splitk_reduce4_kernel<double> "
-                               f"(last GEMM of the matvec, [{4 * D}x{w * D}].[{w * D}x{D}], split-K 4)",
-                     "achieved": round(dom["tflops"], 3), "peak": PEAK["bf16"] / 3.0, "unit": "TFLOP/s",
-                     "frac": round(dom["tflops"] / (PEAK["bf16"] / 3.0), 4),
-                     "traffic": DOMINANT_TRAFFIC_BYTES if (D, w) == (1024, 8) else None,
-                     "note": "3 BF16 MFMA products per algorithmic FLOP: peak = 2500/3 TFLOP/s; traffic = HBM "
-                             "bytes per call from FETCH_SIZE/WRITE_SIZE runs scaled by a 64 MiB streaming kernel"},
+        "roofline": {"bound": "mfma", "kernel": dom_kernel,
+                     "achieved": round(dom["tflops"], 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
+                     "frac": round(dom["tflops"] / PEAK["fp32"], 4),
+                     "traffic": profiled_traffic("fp32_stage4_gemm", D, w),
+                     "note": "achieved = 2*M*N*K / (GEMM + split-K reduction time), HIP events on the library's "
+                             "stream; traffic = HBM bytes per call (FETCH_SIZE x 2 + WRITE_SIZE x 1, scales from a "
+                             "64 MiB streaming kernel) from the counter runs in profiles/"},
         "roofline_fp64": {"bound": "mfma", "achieved": round(gm["fp64"]["tflops"], 3), "peak": PEAK["fp64"],
-                          "unit": "TFLOP/s", "frac": round(gm["fp64"]["tflops"] / PEAK["fp64"], 4)},
+                          "unit": "TFLOP/s", "frac": round(gm["fp64"]["tflops"] / PEAK["fp64"], 4),
+                          "traffic": profiled_traffic("fp64_stage4_gemm", D, w)},
+        "roofline_bf16x3": {"bound": "mfma", "achieved": round(gm["bf16x3"]["tflops"], 3), "peak": PEAK["bf16"],
+                            "unit": "TFLOP/s", "frac": round(gm["bf16x3"]["tflops"] / PEAK["bf16"], 4),
+                            "mfma_issue_frac": round(3.0 * gm["bf16x3"]["tflops"] / PEAK["bf16"], 4),
+                            "traffic": profiled_traffic("bf16x3_stage4_gemm", D, w),
+                            "note": "achieved counts the algorithmic 2*M*N*K once although three BF16 products "
+                                    "run per multiply-add; mfma_issue_frac counts all three against the dense "
+                                    "BF16 peak"},
         "sweep": sweep,
         "eigensolver": {"workload": bench_eigensolver.__doc__.replace("\n", " ").replace("    ", " "), "by_n": eig},
         "matvec_grid": {"workload": bench_matvec_grid.__doc__.replace("\n", " ").replace("    ", " "), "by_w_D": mgrid},

@@ -6,8 +6,11 @@ library or a GPU is missing every entry point raises.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import itertools
 import os
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdmrg_hip.so")
@@ -116,6 +119,39 @@ def call(name: str, *args) -> None:
     if rc != 0:
         msg = lib.ct_last_error().decode(errors="replace")
         raise CTError(name, rc, msg)
+
+
+# Library objects still open (handles, prepared operators, DMRG states), in
+# creation order.  At interpreter exit they are closed newest first - every
+# operator and sweep state before the handle it was created on - while the HIP
+# runtime and torch's allocator are still fully alive, instead of in whatever
+# order garbage collection reaches them during shutdown.
+_live: dict[int, weakref.ref] = {}
+_seq = itertools.count()
+
+
+def track(obj) -> None:
+    """Register an object with a close() method for the ordered teardown at exit."""
+    k = next(_seq)
+    obj._ct_track_key = k
+    _live[k] = weakref.ref(obj)
+
+
+def untrack(obj) -> None:
+    _live.pop(getattr(obj, "_ct_track_key", None), None)
+
+
+@atexit.register
+def _close_all() -> None:
+    for k in sorted(_live, reverse=True):
+        o = _live.get(k)
+        o = o() if o is not None else None
+        if o is not None:
+            try:
+                o.close()
+            except Exception:
+                pass
+    _live.clear()
 
 
 def last_error() -> str:

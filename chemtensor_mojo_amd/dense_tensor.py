@@ -48,6 +48,7 @@ class DeviceContext:
         h = ctypes.c_void_p()
         call("ct_handle_create", device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h))
         self.handle = h
+        _lib.track(self)
 
     def synchronize(self) -> None:
         call("ct_synchronize", self.handle)
@@ -56,6 +57,7 @@ class DeviceContext:
         if self.handle is not None and self.handle.value:
             _lib.load().ct_handle_destroy(self.handle)
             self.handle = None
+        _lib.untrack(self)
 
     def __enter__(self):
         return self

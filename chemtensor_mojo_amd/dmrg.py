@@ -149,6 +149,7 @@ class HeffOperator:
         call("ct_heff_create", ctx.handle, p, L_env.dtype_code, L_env.ptr(), W_i.tensor.ptr(), W_ip1.tensor.ptr(),
              R_env.ptr(), Dl, Dlb, d, Dr, Drb, wL, wM, wR, ctypes.byref(op))
         self.op = op
+        _lib.track(self)
 
     def apply(self, theta: DenseTensor, out: torch.Tensor | None = None) -> DenseTensor:
         Dl, Dlb, d, Dr, Drb = self.dims
@@ -163,6 +164,7 @@ class HeffOperator:
         if getattr(self, "op", None) is not None and self.op.value:
             _lib.load().ct_heff_destroy(self.op)
             self.op = None
+        _lib.untrack(self)
 
     def __del__(self):
         try:
@@ -290,6 +292,7 @@ class DMRGRun:
         call("ct_dmrg_create", ctx.handle, ctypes.byref(p), N, self.d, _lib.F64 if self.dtype == torch.float64
              else _lib.F32, wd, wp, md, mp, ctypes.byref(st))
         self.state = st
+        _lib.track(self)
 
     def run(self, num_sweeps: int = 0) -> tuple[float, list[float]]:
         n = num_sweeps if num_sweeps > 0 else self.params.num_sweeps
@@ -324,6 +327,7 @@ class DMRGRun:
         if self.state is not None and self.state.value:
             _lib.load().ct_dmrg_destroy(self.state)
             self.state = None
+        _lib.untrack(self)
 
     def __del__(self):
         try:
