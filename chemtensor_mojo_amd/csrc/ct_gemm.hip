@@ -6,14 +6,18 @@
 // in column lane%16 (rows: see Mfma<T>::row).  The 3xBF16 path (ct_gemm_bf16.hip)
 // uses v_mfma_f32_16x16x32_bf16 with a k-contiguous LDS layout.
 //
-// Block tile 128x128 (64x64 for small problems), K-tile 16, 256 threads =
-// 4 waves in a 2x2 grid, each wave owning a 64x64 (32x32) sub-tile = 4x4 (2x2)
-// MFMA tiles.  Operand tiles are staged in
-// LDS k-major (sX[k][m]) so every fragment read is 16 consecutive elements per
-// k row.  The next K-tile is prefetched into registers while the MFMAs of the
-// current one run.  Operands may be m/n-contiguous or k-contiguous (the
-// transposes the reference performs with separate permute kernels,
-// dense_tensor.mojo:1249-1257, happen here inside the tile load).
+// Block tile 128x128 (64x64 for small problems), K-tile 16 (FP64) or 32
+// (FP32), 256 threads = 4 waves in a 2x2 grid, each wave owning a 64x64
+// (32x32) sub-tile = 4x4 (2x2) MFMA tiles.  Operands may be m/n-contiguous or
+// k-contiguous (the transposes the reference performs with separate permute
+// kernels, dense_tensor.mojo:1249-1257, happen here inside the tile load).
+// An m/n-contiguous operand is staged in LDS k-major (sX[k][x]), a
+// k-contiguous one x-major (sX[x][k], rows padded to 16-byte multiples):
+// either way the tile store writes each thread's loaded vector contiguously
+// instead of transposing it element by element (measured on the
+// [4096 x 8192] . [8192 x 1024] product: FP32 0.577 -> 0.530 ms, FP64
+// 1.052 -> 1.035 ms).  The next K-tile is prefetched into registers while
+// the MFMAs of the current one run.
 #include <algorithm>
 
 #include "ct_internal.h"
@@ -57,13 +61,19 @@ struct KArgs {
     double alpha, beta;
     const int* active;
     int splits;  // split-K factor (blockIdx.z = batch * splits + split)
-    int kchunk;  // K range per split, a multiple of BK
+    int kchunk;  // K range per split, a multiple of the kernel's K step
     void* part;  // [batch*splits][M][N] partial sums (TC) when splits > 1
 };
 
 // Output tiles of BT x BT (BT = 128, or 64 for problems that 128-wide tiles
-// cannot spread over the CUs), K steps of BK.
+// cannot spread over the CUs), K steps of KStep<TC>: 16 for FP64, 32 for FP32
+// (whose MFMA takes half the cycles per K element; with 32 the per-step tile
+// store and barrier are spread over as many MFMA cycles as in FP64).  BK is
+// the unit of the split-K decision.
 constexpr int BK = 16;
+template <typename TC> struct KStep { static constexpr int value = sizeof(TC) == 8 ? 16 : 32; };
+// padding of an x-major LDS row (sX[x][k]): keeps rows 16-byte aligned
+template <typename TC> struct KPad { static constexpr int value = sizeof(TC) == 8 ? 2 : 4; };
 
 // Loads 4 consecutive elements (16-byte aligned when vec) and converts.
 template <typename TS, typename TD>
@@ -103,7 +113,7 @@ __device__ __forceinline__ void load16(const TS* p, TD* o) {
 // and 16-byte loads are legal (no checks).  For KC, the BK elements of one
 // row are read by BK / VE adjacent lanes (VE elements = 16 bytes each), so
 // one load instruction covers whole rows' K slices.
-template <typename TS, typename TC, bool KC, bool FULL, int BT>
+template <typename TS, typename TC, bool KC, bool FULL, int BT, int BK>
 __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, int X0, int XN, int k0, int K,
                                           bool vec, TC* r) {
     const int tid = threadIdx.x;
@@ -141,24 +151,29 @@ __device__ __forceinline__ void load_tile(const TS* X, int64_t sx, int64_t sk, i
     }
 }
 
-template <typename TS, typename TC, bool KC, int LD, int BT>
-__device__ __forceinline__ void store_tile(TC (*S)[LD], const TC* r) {
+// k-major store of an m/n-contiguous operand: S[k][x]
+template <typename TC, int LD, int BT, int BK>
+__device__ __forceinline__ void store_tile_k(TC (*S)[LD], const TC* r) {
     const int tid = threadIdx.x;
-    if (!KC) {
-        constexpr int CPR = BT / 4, RPP = 256 / CPR, PASSES = BK / RPP;
-        const int c = tid % CPR, row = tid / CPR;
+    constexpr int CPR = BT / 4, RPP = 256 / CPR, PASSES = BK / RPP;
+    const int c = tid % CPR, row = tid / CPR;
 #pragma unroll
-        for (int h = 0; h < PASSES; ++h)
+    for (int h = 0; h < PASSES; ++h)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) S[row + RPP * h][4 * c + e] = r[4 * h + e];
-    } else {
-        constexpr int VE = 16 / (int)sizeof(TS), LPR = BK / VE, RPP = 256 / LPR, PASSES = BT / RPP;
-        const int kq = tid % LPR, xr = tid / LPR;
+        for (int e = 0; e < 4; ++e) S[row + RPP * h][4 * c + e] = r[4 * h + e];
+}
+
+// x-major store of a k-contiguous operand: S[x][k], each thread's VE
+// consecutive k written contiguously
+template <typename TS, typename TC, int LD, int BT, int BK>
+__device__ __forceinline__ void store_tile_x(TC (*S)[LD], const TC* r) {
+    const int tid = threadIdx.x;
+    constexpr int VE = 16 / (int)sizeof(TS), LPR = BK / VE, RPP = 256 / LPR, PASSES = BT / RPP;
+    const int kq = tid % LPR, xr = tid / LPR;
 #pragma unroll
-        for (int h = 0; h < PASSES; ++h)
+    for (int h = 0; h < PASSES; ++h)
 #pragma unroll
-            for (int e = 0; e < VE; ++e) S[VE * kq + e][xr + RPP * h] = r[VE * h + e];
-    }
+        for (int e = 0; e < VE; ++e) S[xr + RPP * h][VE * kq + e] = r[VE * h + e];
 }
 
 // LDS is double-buffered: the tile for k0+BK is loaded into registers before
@@ -167,11 +182,15 @@ __device__ __forceinline__ void store_tile(TC (*S)[LD], const TC* r) {
 template <typename TC, typename TA, typename TB, typename TO, bool AKC, bool BKC, bool FULL, int BT>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     if (p.active && *p.active == 0) return;
-    constexpr int PAD = 4;
+    constexpr int BK = KStep<TC>::value;
+    constexpr int PAD = 4, KP = BK + KPad<TC>::value;
     constexpr int NI = BT / 32;   // 16x16 accumulator blocks per wave and dimension (2 x 2 waves)
     constexpr int NR = BT * BK / 256;
-    __shared__ TC sA[2][BK][BT + PAD];
-    __shared__ TC sB[2][BK][BT + PAD];
+    // k-major (x-contiguous operand) or x-major (k-contiguous operand) LDS tiles
+    __shared__ __attribute__((aligned(16))) TC sAk[AKC ? 1 : 2][AKC ? 1 : BK][BT + PAD];
+    __shared__ __attribute__((aligned(16))) TC sAx[AKC ? 2 : 1][AKC ? BT : 1][KP];
+    __shared__ __attribute__((aligned(16))) TC sBk[BKC ? 1 : 2][BKC ? 1 : BK][BT + PAD];
+    __shared__ __attribute__((aligned(16))) TC sBx[BKC ? 2 : 1][BKC ? BT : 1][KP];
     typedef typename Mfma<TC>::acc_t acc_t;
 
     int tm, tn;
@@ -196,37 +215,46 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
         for (int j = 0; j < NI; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     TC ra[NR], rb[NR];
-    const int64_t sAx = AKC ? p.sAm : 0, sAk = AKC ? 0 : p.sAk;
-    const int64_t sBx = BKC ? p.sBn : 0, sBk = BKC ? 0 : p.sBk;
-    load_tile<TA, TC, AKC, FULL, BT>(A, sAx, sAk, m0, p.M, kbeg, kend, p.vecA, ra);
-    load_tile<TB, TC, BKC, FULL, BT>(B, sBx, sBk, n0, p.N, kbeg, kend, p.vecB, rb);
-    store_tile<TA, TC, AKC, BT + PAD, BT>(sA[0], ra);
-    store_tile<TB, TC, BKC, BT + PAD, BT>(sB[0], rb);
+    const int64_t sAx_ = AKC ? p.sAm : 0, sAk_ = AKC ? 0 : p.sAk;
+    const int64_t sBx_ = BKC ? p.sBn : 0, sBk_ = BKC ? 0 : p.sBk;
+    auto store = [&](int buf) {
+        if constexpr (AKC) store_tile_x<TA, TC, KP, BT, BK>(sAx[buf], ra);
+        else store_tile_k<TC, BT + PAD, BT, BK>(sAk[buf], ra);
+        if constexpr (BKC) store_tile_x<TB, TC, KP, BT, BK>(sBx[buf], rb);
+        else store_tile_k<TC, BT + PAD, BT, BK>(sBk[buf], rb);
+    };
+    load_tile<TA, TC, AKC, FULL, BT, BK>(A, sAx_, sAk_, m0, p.M, kbeg, kend, p.vecA, ra);
+    load_tile<TB, TC, BKC, FULL, BT, BK>(B, sBx_, sBk_, n0, p.N, kbeg, kend, p.vecB, rb);
+    store(0);
     __syncthreads();
 
     int cur = 0;
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
         const bool more = k0 + BK < kend;
         if (more) {
-            load_tile<TA, TC, AKC, FULL, BT>(A, sAx, sAk, m0, p.M, k0 + BK, kend, p.vecA, ra);
-            load_tile<TB, TC, BKC, FULL, BT>(B, sBx, sBk, n0, p.N, k0 + BK, kend, p.vecB, rb);
+            load_tile<TA, TC, AKC, FULL, BT, BK>(A, sAx_, sAk_, m0, p.M, k0 + BK, kend, p.vecA, ra);
+            load_tile<TB, TC, BKC, FULL, BT, BK>(B, sBx_, sBk_, n0, p.N, k0 + BK, kend, p.vecB, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK / 4; ++kk) {
             TC a[NI], b[NI];
+            const int k = kk * 4 + lk;
 #pragma unroll
-            for (int i = 0; i < NI; ++i) a[i] = sA[cur][kk * 4 + lk][wm * (BT / 2) + i * 16 + li];
+            for (int i = 0; i < NI; ++i) {
+                const int x = wm * (BT / 2) + i * 16 + li;
+                a[i] = AKC ? sAx[cur][x][k] : sAk[cur][k][x];
+            }
 #pragma unroll
-            for (int j = 0; j < NI; ++j) b[j] = sB[cur][kk * 4 + lk][wn * (BT / 2) + j * 16 + li];
+            for (int j = 0; j < NI; ++j) {
+                const int x = wn * (BT / 2) + j * 16 + li;
+                b[j] = BKC ? sBx[cur][x][k] : sBk[cur][k][x];
+            }
 #pragma unroll
             for (int i = 0; i < NI; ++i)
 #pragma unroll
                 for (int j = 0; j < NI; ++j) acc[i][j] = Mfma<TC>::op(a[i], b[j], acc[i][j]);
         }
-        if (more) {
-            store_tile<TA, TC, AKC, BT + PAD, BT>(sA[cur ^ 1], ra);
-            store_tile<TB, TC, BKC, BT + PAD, BT>(sB[cur ^ 1], rb);
-        }
+        if (more) store(cur ^ 1);
         __syncthreads();
         cur ^= 1;
     }
@@ -298,14 +326,14 @@ bool aligned16(const void* ptr, int64_t s1, int64_t s2) {
 // lone workgroup has nothing to run while it waits at its per-step
 // barrier (the [4096 x 8192] . [8192 x 1024] product of the D = 1024
 // matvec is 256 tiles of 128 x 128).
-void plan_splits(const GemmDesc& g, int BT, int* splits_out, int* kchunk_out) {
+void plan_splits(const GemmDesc& g, int BT, int BKT, int* splits_out, int* kchunk_out) {
     const int64_t tiles = (int64_t)((g.M + BT - 1) / BT) * ((g.N + BT - 1) / BT) * g.batch;
     int splits = 1;
     if (tiles < 128 && g.K >= 16 * BK) splits = (int)std::min<int64_t>((256 + tiles - 1) / tiles, g.K / (8 * BK));
     else if (tiles < 512 && g.K >= 128 * BK) splits = (int)std::min<int64_t>((512 + tiles - 1) / tiles, g.K / (64 * BK));
     int kchunk = g.K;
     if (splits > 1) {
-        kchunk = (int)(((g.K + splits - 1) / splits + BK - 1) / BK * BK);
+        kchunk = (int)(((g.K + splits - 1) / splits + BKT - 1) / BKT * BKT);  // whole K steps of the kernel
         splits = (g.K + kchunk - 1) / kchunk;
     }
     *splits_out = splits;
@@ -337,12 +365,13 @@ int launch_bt(Handle* h, const void* A, const void* B, void* C, const GemmDesc& 
     p.vecA = akc ? aligned16<TA>(A, g.sAm, g.bsA) : aligned16<TA>(A, g.sAk, g.bsA);
     p.vecB = bkc ? aligned16<TB>(B, g.sBn, g.bsB) : aligned16<TB>(B, g.sBk, g.bsB);
     int splits = 1;
-    plan_splits(g, BT, &splits, &p.kchunk);
+    constexpr int BKT = KStep<TC>::value;
+    plan_splits(g, BT, BKT, &splits, &p.kchunk);
     p.splits = splits;
     p.part = nullptr;
     if (splits > 1) CT_CHECK(ws_get(h, WS_SPLITK, (size_t)g.batch * splits * g.M * g.N * sizeof(TC), &p.part));
     dim3 grid(p.tilesM * p.tilesN, 1, g.batch * splits), block(256);
-    const bool full = p.vecA && p.vecB && (g.M % BT == 0) && (g.N % BT == 0) && (g.K % BK == 0);
+    const bool full = p.vecA && p.vecB && (g.M % BT == 0) && (g.N % BT == 0) && (g.K % BKT == 0);
 #define CT_LAUNCH(AK, BKc, F) \
     hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, AK, BKc, F, BT>), grid, block, 0, h->stream, p)
     if (full) {
@@ -427,7 +456,7 @@ int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int 
 size_t gemm_splitk_bytes(int prec, const GemmDesc& g) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0 || g.K <= 0) return 0;
     int splits = 1, kchunk = 0;
-    plan_splits(g, plan_tile(g), &splits, &kchunk);
+    plan_splits(g, plan_tile(g), prec == CT_PREC_FP64 ? KStep<double>::value : KStep<float>::value, &splits, &kchunk);
     if (splits <= 1) return 0;
     const size_t es = prec == CT_PREC_FP64 ? 8 : 4;
     return (size_t)g.batch * splits * g.M * g.N * es;
