@@ -17,7 +17,9 @@ events on the stream it runs on; fixed-D D=1024 DMRG sweeps in FP64 and in
 the refined FP32 / 3xBF16 forms on two chains - one whose local problems are
 converged after the first sweep (Majumdar-Ghosh point) and one whose Lanczos
 runs its whole Krylov space (antiferromagnetic Heisenberg); the symmetric
-eigensolver of the truncation; the matvec grid over D and w; and the FP64 CPU
+eigensolver of the truncation; the matvec grid over D and w; whole DMRG runs
+of two benchmark configurations with per-stage times (XXZ antiferromagnet L=64,
+chi <= 256, FP64; J1-J2 L=100, chi <= 1024, all three forms); and the FP64 CPU
 oracle timed on the host cores.
 
 Prints one JSON line.  N > 1 GPUs is not supported (the path runs on one GPU).
@@ -291,21 +293,21 @@ def _sweep_deltas(run, nsweeps, tag):
     return out
 
 
-def study_stages(ctx, args):
-    """Plain FP64 DMRG on the GPU next to the FP64 CPU oracle: Heisenberg XXZ
-    chain L=64 (antiferromagnetic sign J=-1 of H = -J sum(XX+YY+Delta ZZ), so the
-    bond dimension grows), Delta=1, w=5, PCG32 random start D=16, chi_max=256,
-    eps_trunc=1e-10, 6 sweeps, 20 Krylov vectors with converged tridiagonal
-    solves (the reference's 200-rotation cap stalls 20-vector runs at
-    rounding-dependent energies); stage times per sweep.  The oracle runs the
-    same sweeps on all host cores."""
+def config2_sweeps(ctx, modes):
+    """Heisenberg XXZ chain L=64 (antiferromagnetic sign J=-1 of
+    H = -J sum(XX+YY+Delta ZZ), so the bond dimension grows), Delta=1, w=5,
+    PCG32 random start D=16, chi_max=256, eps_trunc=1e-10, 6 FP64 sweeps, 20
+    Krylov vectors with converged tridiagonal solves (the reference's
+    200-rotation cap stalls 20-vector runs at rounding-dependent energies);
+    per sweep: wall time, Lanczos (H_eff matvecs and the tridiagonal solves),
+    truncated split and environment update times."""
     import chemtensor_mojo_amd as ct
     from chemtensor_mojo_amd import hamiltonians as ham
     L, chi, sweeps = 64, 256, 6
     mpo_t = ham.heisenberg_xxz_tensors(L, -1.0, 1.0, 0.0)
     mps_t = ct.random_mps_c_compatible_tensors(L, 2, 16, 42)
     res = {}
-    for mode, name in ((ct.SVD_GRAM, "gpu_fp64_gram"), (ct.SVD_GESVD, "gpu_fp64_gesvd")):
+    for mode, name in modes:
         p = ct.DMRGParams(num_sweeps=1, chi_max=chi, eps_trunc=1e-10, max_krylov_iter=20, krylov_tol=1e-10,
                           energy_tol=0.0, svd_mode=mode, jacobi_max_rot=-1)
         run = ct.DMRGRun(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p)
@@ -313,33 +315,12 @@ def study_stages(ctx, args):
             res[name] = _sweep_deltas(run, sweeps, name)
         finally:
             run.close()
-    cpu = None
-    if not args.no_cpu:
-        from oracle import oracle as ora
-        ora.build()
-        ora.set_num_threads(os.cpu_count() or 1)
-        cur, rows = mps_t, []
-        for sw in range(sweeps):  # one sweep per call, each restarting from the previous result
-            t0 = time.perf_counter()
-            ref = ora.dmrg_two_site(cur, mpo_t, 1, chi, 1e-10, 20, 1e-10, 0.0, jacobi_max_rot=20000)
-            tc = time.perf_counter() - t0
-            cur = ref["mps"]
-            rows.append({"sweep": sw, "s": tc, "E": float(ref["sweep_E"][0]),
-                         "max_bond": int(max(a.shape[2] for a in cur))})
-            log(f"cpu oracle sweep {sw}: {tc:.2f} s E={rows[-1]['E']:.12f} maxD={rows[-1]['max_bond']}")
-        cpu = {"value": rows[-1]["s"], "unit": "s (last sweep)", "cores": ora.get_num_threads(), "kind": "port",
-               "sample": f"{sweeps} sweeps of the same run, FP64 oracle (OpenBLAS dgemm + LAPACKE dgesdd), "
-                         "one oracle call per sweep", "sweeps": rows}
-    g = res["gpu_fp64_gram"]
-    line = {"study": "fp64_stages", "workload": study_stages.__doc__.split("\n\n")[0].replace("\n", " "),
-            "sweeps": res, "cpu_oracle": cpu,
-            "speedup_last_sweep_vs_cpu": (cpu["value"] / g[-1]["s"]) if cpu else None}
-    print(json.dumps(line), flush=True)
+    return mpo_t, mps_t, res
 
 
-def study_precision(ctx, args):
-    """Precision study: J1-J2 chain (J1=1, J2=0.5, w=8) L=100, PCG32 random start
-    D=32, chi_max=1024, eps_trunc=1e-12, 8 sweeps, 20 Krylov vectors, converged
+def config3_sweeps(ctx):
+    """J1-J2 chain (J1=1, J2=0.5, w=8) L=100, PCG32 random start D=32,
+    chi_max=1024, eps_trunc=1e-12, 8 sweeps, 20 Krylov vectors, converged
     tridiagonal solves; FP64 vs FP32 and 3xBF16 matvecs with FP64 refinement.
     J2=0.5 is the Majumdar-Ghosh point, E0 = -3L/2 = -150 exactly."""
     import chemtensor_mojo_amd as ct
@@ -362,13 +343,53 @@ def study_precision(ctx, args):
     summary = {}
     for k, v in res.items():
         tw = sum(r["trunc_weight_sum"] for r in v)
+        rise = max([0.0] + [v[i + 1]["E"] - v[i]["E"] for i in range(len(v) - 1)])
         summary[k] = {"E": v[-1]["E"], "dE_rel_vs_fp64": abs(v[-1]["E"] - E64) / abs(E64),
                       "dE_rel_vs_exact": abs(v[-1]["E"] + 1.5 * L) / (1.5 * L),
+                      "max_sweep_rise_rel": rise / (1.5 * L),
                       "trunc_weight_sum": tw, "d_trunc_weight_vs_fp64": tw - tw64,
-                      "total_s": sum(r["s"] for r in v)}
+                      "max_bond": v[-1]["max_bond"], "total_s": sum(r["s"] for r in v),
+                      "lanczos_s": sum(r["t_lanczos"] for r in v), "svd_s": sum(r["t_svd"] for r in v)}
         log(f"{k}: E={v[-1]['E']:.14f} dE/E vs fp64 {summary[k]['dE_rel_vs_fp64']:.2e} "
             f"vs exact {summary[k]['dE_rel_vs_exact']:.2e} tw={tw:.3e} time={summary[k]['total_s']:.2f} s")
-    print(json.dumps({"study": "precision", "workload": study_precision.__doc__.replace("\n", " "),
+    return summary, res
+
+
+def study_stages(ctx, args):
+    """Plain FP64 DMRG on the GPU next to the FP64 CPU oracle (both split
+    modes on the GPU); the oracle runs the same sweeps on all host cores."""
+    import chemtensor_mojo_amd as ct
+    sweeps = 6
+    mpo_t, mps_t, res = config2_sweeps(ctx, ((ct.SVD_GRAM, "gpu_fp64_gram"), (ct.SVD_GESVD, "gpu_fp64_gesvd")))
+    chi = 256
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as ora
+        ora.build()
+        ora.set_num_threads(os.cpu_count() or 1)
+        cur, rows = mps_t, []
+        for sw in range(sweeps):  # one sweep per call, each restarting from the previous result
+            t0 = time.perf_counter()
+            ref = ora.dmrg_two_site(cur, mpo_t, 1, chi, 1e-10, 20, 1e-10, 0.0, jacobi_max_rot=20000)
+            tc = time.perf_counter() - t0
+            cur = ref["mps"]
+            rows.append({"sweep": sw, "s": tc, "E": float(ref["sweep_E"][0]),
+                         "max_bond": int(max(a.shape[2] for a in cur))})
+            log(f"cpu oracle sweep {sw}: {tc:.2f} s E={rows[-1]['E']:.12f} maxD={rows[-1]['max_bond']}")
+        cpu = {"value": rows[-1]["s"], "unit": "s (last sweep)", "cores": ora.get_num_threads(), "kind": "port",
+               "sample": f"{sweeps} sweeps of the same run, FP64 oracle (OpenBLAS dgemm + LAPACKE dgesdd), "
+                         "one oracle call per sweep", "sweeps": rows}
+    g = res["gpu_fp64_gram"]
+    line = {"study": "fp64_stages", "workload": " ".join((config2_sweeps.__doc__ + " " + study_stages.__doc__).split()),
+            "sweeps": res, "cpu_oracle": cpu,
+            "speedup_last_sweep_vs_cpu": (cpu["value"] / g[-1]["s"]) if cpu else None}
+    print(json.dumps(line), flush=True)
+
+
+def study_precision(ctx, args):
+    """Precision study (the same runs as the default line's config-3 entry)."""
+    summary, res = config3_sweeps(ctx)
+    print(json.dumps({"study": "precision", "workload": " ".join(config3_sweeps.__doc__.split()),
                       "summary": summary, "sweeps": res}), flush=True)
 
 
@@ -687,6 +708,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the eigensolver and matvec-grid sections")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the L=64 XXZ (chi 256) and L=100 J1-J2 (chi 1024) sweep sections")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--qc-sweeps", type=int, default=6)
     ap.add_argument("--qc-penalty", type=float, default=1.0)
@@ -739,6 +762,19 @@ def main():
                                                                runs["bf16x3_refined"]["sweep_s"])],
                 "speedup_sweep_fp32": [a / b for a, b in zip(runs["fp64"]["sweep_s"], runs["fp32_refined"]["sweep_s"])],
             }
+    configs = None
+    if not args.no_configs:
+        _, _, c2 = config2_sweeps(ctx, ((ct.SVD_GRAM, "fp64_gram"),))
+        c3_summary, c3 = config3_sweeps(ctx)
+        stage_keys = ("s", "t_lanczos", "t_svd", "t_env", "matvecs", "max_bond", "E")
+        configs = {
+            "xxz_afm_L64_chi256": {"workload": " ".join(config2_sweeps.__doc__.split()),
+                                   "per_sweep": [{k: r[k] for k in stage_keys} for r in c2["fp64_gram"]],
+                                   "total_s": sum(r["s"] for r in c2["fp64_gram"])},
+            "j1j2_L100_chi1024": {"workload": " ".join(config3_sweeps.__doc__.split()), "summary": c3_summary,
+                                  "per_sweep": {k: [{kk: r[kk] for kk in stage_keys} for r in v]
+                                                for k, v in c3.items()}},
+        }
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(D, w, args.cpu_budget)
@@ -785,6 +821,7 @@ def main():
                                     "run per multiply-add; mfma_issue_frac counts all three against the dense "
                                     "BF16 peak"},
         "sweep": sweep,
+        "configs": configs,
         "eigensolver": {"workload": bench_eigensolver.__doc__.replace("\n", " ").replace("    ", " "), "by_n": eig},
         "matvec_grid": {"workload": bench_matvec_grid.__doc__.replace("\n", " ").replace("    ", " "), "by_w_D": mgrid},
         "cpu_baseline": cpu,

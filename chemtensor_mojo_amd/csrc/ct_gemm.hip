@@ -6,8 +6,8 @@
 // in column lane%16 (rows: see Mfma<T>::row).  The 3xBF16 path (ct_gemm_bf16.hip)
 // uses v_mfma_f32_16x16x32_bf16 with a k-contiguous LDS layout.
 //
-// Block tile 128x128 (64x64 for small problems), K-tile 16 (FP64) or 32
-// (FP32), 256 threads = 4 waves in a 2x2 grid, each wave owning a 64x64
+// Block tile 128x128 (64x64 for small problems), K-tile 16 (FP64, guarded
+// FP32 tiles) or 32 (unguarded FP32 tiles), 256 threads = 4 waves in a 2x2 grid, each wave owning a 64x64
 // (32x32) sub-tile = 4x4 (2x2) MFMA tiles.  Operands may be m/n-contiguous or
 // k-contiguous (the transposes the reference performs with separate permute
 // kernels, dense_tensor.mojo:1249-1257, happen here inside the tile load).
@@ -66,12 +66,14 @@ struct KArgs {
 };
 
 // Output tiles of BT x BT (BT = 128, or 64 for problems that 128-wide tiles
-// cannot spread over the CUs), K steps of KStep<TC>: 16 for FP64, 32 for FP32
-// (whose MFMA takes half the cycles per K element; with 32 the per-step tile
-// store and barrier are spread over as many MFMA cycles as in FP64).  BK is
-// the unit of the split-K decision.
+// cannot spread over the CUs), K steps of 16, or 32 for FP32 products whose
+// tiles need no bounds checks (the FP32 MFMA takes half the cycles per K
+// element, and with 32 the per-step tile store and barrier are spread over as
+// many MFMA cycles as in FP64).  Guarded FP32 tiles keep 16: at the 2248 x
+// 2004 x 512 MPO-mixing products of a w = 562 bond, 32 ran 22.0 ms against
+// 18.1 ms.  BK is the unit of the split-K decision.
 constexpr int BK = 16;
-template <typename TC> struct KStep { static constexpr int value = sizeof(TC) == 8 ? 16 : 32; };
+constexpr int kStepFp32Full = 32;
 // padding of an x-major LDS row (sX[x][k]): keeps rows 16-byte aligned
 template <typename TC> struct KPad { static constexpr int value = sizeof(TC) == 8 ? 2 : 4; };
 
@@ -179,10 +181,10 @@ __device__ __forceinline__ void store_tile_x(TC (*S)[LD], const TC* r) {
 // LDS is double-buffered: the tile for k0+BK is loaded into registers before
 // the MFMAs on k0 and stored to the other buffer after them, so each K step
 // needs a single barrier.
-template <typename TC, typename TA, typename TB, typename TO, bool AKC, bool BKC, bool FULL, int BT>
+template <typename TC, typename TA, typename TB, typename TO, bool AKC, bool BKC, bool FULL, int BT, int KSTEP>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(KArgs p) {
     if (p.active && *p.active == 0) return;
-    constexpr int BK = KStep<TC>::value;
+    constexpr int BK = KSTEP;
     constexpr int PAD = 4, KP = BK + KPad<TC>::value;
     constexpr int NI = BT / 32;   // 16x16 accumulator blocks per wave and dimension (2 x 2 waves)
     constexpr int NR = BT * BK / 256;
@@ -340,6 +342,12 @@ void plan_splits(const GemmDesc& g, int BT, int BKT, int* splits_out, int* kchun
     *kchunk_out = kchunk;
 }
 
+// K step of the kernel: kStepFp32Full for FP32 (elem_size 4) when the tiles
+// need no bounds checks and K is a multiple of it, else BK.
+int kstep_for(size_t elem_size, bool tiles_full, int64_t K) {
+    return (elem_size == 4 && tiles_full && K % kStepFp32Full == 0) ? kStepFp32Full : BK;
+}
+
 // Tile size of the FP64/FP32 GEMM: 64x64 when 128x128 tiles would occupy
 // fewer than half the CUs and split-K cannot make up for it (K < 2048, or
 // fewer than 16 tiles): four times the workgroups for the same problem.  With
@@ -364,26 +372,34 @@ int launch_bt(Handle* h, const void* A, const void* B, void* C, const GemmDesc& 
     const bool bkc = g.b_kc != 0;
     p.vecA = akc ? aligned16<TA>(A, g.sAm, g.bsA) : aligned16<TA>(A, g.sAk, g.bsA);
     p.vecB = bkc ? aligned16<TB>(B, g.sBn, g.bsB) : aligned16<TB>(B, g.sBk, g.bsB);
+    const bool tiles_full = p.vecA && p.vecB && (g.M % BT == 0) && (g.N % BT == 0);
+    const int BKT = kstep_for(sizeof(TC), tiles_full, g.K);
     int splits = 1;
-    constexpr int BKT = KStep<TC>::value;
     plan_splits(g, BT, BKT, &splits, &p.kchunk);
     p.splits = splits;
     p.part = nullptr;
     if (splits > 1) CT_CHECK(ws_get(h, WS_SPLITK, (size_t)g.batch * splits * g.M * g.N * sizeof(TC), &p.part));
     dim3 grid(p.tilesM * p.tilesN, 1, g.batch * splits), block(256);
-    const bool full = p.vecA && p.vecB && (g.M % BT == 0) && (g.N % BT == 0) && (g.K % BKT == 0);
-#define CT_LAUNCH(AK, BKc, F) \
-    hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, AK, BKc, F, BT>), grid, block, 0, h->stream, p)
-    if (full) {
-        if (!akc && !bkc) CT_LAUNCH(false, false, true);
-        else if (akc && !bkc) CT_LAUNCH(true, false, true);
-        else if (!akc && bkc) CT_LAUNCH(false, true, true);
-        else CT_LAUNCH(true, true, true);
+    const bool full = tiles_full && (g.K % BKT == 0);
+#define CT_LAUNCH(AK, BKc, F, KS) \
+    hipLaunchKernelGGL((gemm_mfma_kernel<TC, TA, TB, TO, AK, BKc, F, BT, KS>), grid, block, 0, h->stream, p)
+    if (BKT == kStepFp32Full) {  // FP32, unguarded tiles (kstep_for returns this only then)
+        if constexpr (sizeof(TC) == 4) {
+            if (!akc && !bkc) CT_LAUNCH(false, false, true, kStepFp32Full);
+            else if (akc && !bkc) CT_LAUNCH(true, false, true, kStepFp32Full);
+            else if (!akc && bkc) CT_LAUNCH(false, true, true, kStepFp32Full);
+            else CT_LAUNCH(true, true, true, kStepFp32Full);
+        }
+    } else if (full) {
+        if (!akc && !bkc) CT_LAUNCH(false, false, true, BK);
+        else if (akc && !bkc) CT_LAUNCH(true, false, true, BK);
+        else if (!akc && bkc) CT_LAUNCH(false, true, true, BK);
+        else CT_LAUNCH(true, true, true, BK);
     } else {
-        if (!akc && !bkc) CT_LAUNCH(false, false, false);
-        else if (akc && !bkc) CT_LAUNCH(true, false, false);
-        else if (!akc && bkc) CT_LAUNCH(false, true, false);
-        else CT_LAUNCH(true, true, false);
+        if (!akc && !bkc) CT_LAUNCH(false, false, false, BK);
+        else if (akc && !bkc) CT_LAUNCH(true, false, false, BK);
+        else if (!akc && bkc) CT_LAUNCH(false, true, false, BK);
+        else CT_LAUNCH(true, true, false, BK);
     }
 #undef CT_LAUNCH
     CT_HIP(hipGetLastError());
@@ -455,11 +471,17 @@ int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int 
 
 size_t gemm_splitk_bytes(int prec, const GemmDesc& g) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0 || g.K <= 0) return 0;
-    int splits = 1, kchunk = 0;
-    plan_splits(g, plan_tile(g), prec == CT_PREC_FP64 ? KStep<double>::value : KStep<float>::value, &splits, &kchunk);
-    if (splits <= 1) return 0;
+    // the K step also depends on the operands' alignment, known only at launch:
+    // take the larger partial-sum buffer of the two possible steps
     const size_t es = prec == CT_PREC_FP64 ? 8 : 4;
-    return (size_t)g.batch * splits * g.M * g.N * es;
+    size_t bytes = 0;
+    for (int kstep : {BK, kStepFp32Full}) {
+        if (es == 8 && kstep != BK) continue;
+        int splits = 1, kchunk = 0;
+        plan_splits(g, plan_tile(g), kstep, &splits, &kchunk);
+        if (splits > 1) bytes = std::max(bytes, (size_t)g.batch * splits * g.M * g.N * es);
+    }
+    return bytes;
 }
 
 }  // namespace ct
