@@ -41,6 +41,7 @@ sys.path.insert(0, ROOT)
 METRIC = "DMRG sweep s + H_eff matvec TFLOP/s at D=1024; |ΔE/E| vs FP64 oracle"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # TFLOP/s, MI355X dense MFMA
 HBM_PEAK = 8000.0  # GB/s
+WARM_S = 0.25  # untimed sustained work before each dominant-GEMM timing
 
 
 def log(*a):
@@ -67,10 +68,18 @@ def make_problem(D, w, seed=42):
     return L, R, W, th
 
 
-def time_events(fn, steps, warmup):
+def time_events(fn, steps, warmup, min_warm_s=0.0):
+    """Mean seconds per call of fn over `steps` calls (HIP events on the current stream) after `warmup` untimed
+    calls, continued until at least min_warm_s of them have run (the GPU clock settles over the first tens of
+    milliseconds of sustained MFMA work)."""
     import torch
-    for _ in range(warmup):
+    t_w = time.perf_counter()
+    n = 0
+    while n < warmup or time.perf_counter() - t_w < min_warm_s:
         fn()
+        n += 1
+        if min_warm_s > 0.0 and n % 8 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -192,13 +201,13 @@ def bench_dominant_gemm(ctx, D, w, steps, warmup):
     tA = ct.DenseTensor(A64, [M, K])
     tB = ct.DenseTensor(B64, [K, N])
     tC = ct.create_dense_tensor(ctx, [M, N])
-    t, _ = time_events(lambda: ct.dense_tensor_dot(tC, tA, tB, ctx), steps, warmup)
+    t, _ = time_events(lambda: ct.dense_tensor_dot(tC, tA, tB, ctx), steps, warmup, WARM_S)
     out["fp64"] = {"s": t, "tflops": F / t / 1e12, "bytes": 8.0 * (M * K + K * N + M * N)}
     # FP32 (the reference's precision): the same product on float32 copies
     tA32 = ct.DenseTensor(A64.float(), [M, K])
     tB32 = ct.DenseTensor(B64.float(), [K, N])
     tC32 = ct.create_dense_tensor(ctx, [M, N], dtype=torch.float32)
-    t, _ = time_events(lambda: ct.dense_tensor_dot(tC32, tA32, tB32, ctx), steps, warmup)
+    t, _ = time_events(lambda: ct.dense_tensor_dot(tC32, tA32, tB32, ctx), steps, warmup, WARM_S)
     out["fp32"] = {"s": t, "tflops": F / t / 1e12, "bytes": 4.0 * (M * K + K * N + M * N)}
     # 3xBF16: pre-split operands (split rows), time only the NT MFMA kernels
     ld = 2 * ((K + 31) // 32 * 32)
@@ -209,7 +218,7 @@ def bench_dominant_gemm(ctx, D, w, steps, warmup):
     _lib.call("ct_split_bf16", ctx.handle, 0, vp(B64), N, K, 1, N, vp(Bs), ld)
     C32 = torch.empty(M * N, dtype=torch.float64, device="cuda")
     t, _ = time_events(lambda: _lib.call("ct_gemm_bf16x3_nt", ctx.handle, vp(As), ld, vp(Bs), ld, 0, vp(C32), M, N, K,
-                                         N), steps, warmup)
+                                         N), steps, warmup, WARM_S)
     out["bf16x3"] = {"s": t, "tflops": F / t / 1e12, "mfma_tflops": 3 * F / t / 1e12,
                      "bytes": 2.0 * ld * (M + N) + 8.0 * M * N}
     ref = (A64.view(M, K) @ B64.view(K, N)).reshape(-1)
@@ -699,7 +708,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--D", type=int, default=1024)
     ap.add_argument("--w", type=int, default=8)
     ap.add_argument("--sweep-sites", type=int, default=24)
@@ -808,7 +817,7 @@ def main():
                      "frac": round(dom["tflops"] / PEAK["fp32"], 4),
                      "traffic": profiled_traffic("fp32_stage4_gemm", D, w),
                      "note": "achieved = 2*M*N*K / (GEMM + split-K reduction time), HIP events on the library's "
-                             "stream; traffic = HBM bytes per call (FETCH_SIZE x 2 + WRITE_SIZE x 1, scales from a "
+                             "stream, mean of `steps` calls after >= 0.25 s of untimed calls; traffic = HBM bytes per call (FETCH_SIZE x 2 + WRITE_SIZE x 1, scales from a "
                              "64 MiB streaming kernel) from the counter runs in profiles/"},
         "roofline_fp64": {"bound": "mfma", "achieved": round(gm["fp64"]["tflops"], 3), "peak": PEAK["fp64"],
                           "unit": "TFLOP/s", "frac": round(gm["fp64"]["tflops"] / PEAK["fp64"], 4),

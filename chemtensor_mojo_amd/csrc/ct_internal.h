@@ -112,7 +112,7 @@ enum WsSlot {
     WS_TMP4 = 15,
     WS_LSPLIT = 16,  // prepared (converted / split) left environment
     WS_RSPLIT = 17,  // prepared right environment
-    WS_VSPLIT = 18,  // split Krylov vector (3xBF16 matvec)
+    WS_VSPLIT = 18,  // split Krylov vector (3xBF16 matvec) / its FP32 copy (FP32 matvec)
     WS_USPLIT = 19,  // split mixing output (3xBF16 matvec)
     WS_ENV_T1 = 20,  // environment update intermediates
     WS_ENV_T2 = 21,
@@ -214,7 +214,8 @@ int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, cons
 
 // ------------------------------------------------------------- misc kernels
 int permute(Handle* h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm, void* out);
-int convert(Handle* h, int tin, const void* in, int tout, void* out, int64_t n);
+// out = (tout) in, n elements; with `active`, a no-op on the device when *active == 0
+int convert(Handle* h, int tin, const void* in, int tout, void* out, int64_t n, const int* active = nullptr);
 int fill(Handle* h, int dtype, void* x, int64_t n, double value);
 int set_identity_env(Handle* h, int dtype, void* x, int64_t w, int64_t D);
 // reductions into a device FP64 scalar (no host sync)

@@ -96,7 +96,8 @@ extern "C" int ct_heff_apply_noalloc(ct_handle_t h, int prec, int dtype, const v
                 arena_reserve(a, tAcc == CT_F64 ? WS_W2PAD64 : WS_W2PAD32, mix_pad_bytes((int)K, (int)NO, tAcc));
     if (fits && prec == CT_PREC_FP32 && dtype == CT_F64)
         fits = arena_reserve(a, WS_LSPLIT, (size_t)(wL * Dl * Dlb) * 4) &&
-               arena_reserve(a, WS_RSPLIT, (size_t)(wR * Dr * Drb) * 4);
+               arena_reserve(a, WS_RSPLIT, (size_t)(wR * Dr * Drb) * 4) &&
+               arena_reserve(a, WS_VSPLIT, (size_t)(Dl * d2 * Dr) * 4);
     if (fits) fits = arena_reserve(a, WS_T1, (size_t)nT1 * es) && arena_reserve(a, WS_T2, (size_t)nU * es);
     if (fits && sk > 0) fits = arena_reserve(a, WS_SPLITK, sk);
     if (!fits) {

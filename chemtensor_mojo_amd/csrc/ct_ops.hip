@@ -135,9 +135,22 @@ __global__ __launch_bounds__(256) void permute_kernel(const T* __restrict__ in, 
 }
 
 template <typename TI, typename TO>
-__global__ __launch_bounds__(256) void convert_kernel(const TI* __restrict__ in, TO* __restrict__ out, int64_t n) {
+__global__ __launch_bounds__(256) void convert_kernel(const TI* __restrict__ in, TO* __restrict__ out, int64_t n,
+                                                      const int* active) {
+    if (active && *active == 0) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = (TO)in[i];
+}
+
+// FP64 -> FP32 four elements per thread (two 16-byte loads, one 16-byte
+// store); n % 4 == 0 and both pointers 16-byte aligned
+__global__ __launch_bounds__(256) void convert4_kernel(const double2* __restrict__ in, float4* __restrict__ out,
+                                                       int64_t n4, const int* active) {
+    if (active && *active == 0) return;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x) {
+        const double2 a = in[2 * i], b = in[2 * i + 1];
+        out[i] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+    }
 }
 
 template <typename T>
@@ -500,14 +513,22 @@ int permute(Handle* h, int dtype, const void* in, int rank, const int64_t* shape
     return 0;
 }
 
-int convert(Handle* h, int tin, const void* in, int tout, void* out, int64_t n) {
+int convert(Handle* h, int tin, const void* in, int tout, void* out, int64_t n, const int* active) {
     if (n <= 0) return 0;
     dim3 g(grid_for(n)), b(256);
-    if (tin == CT_F64 && tout == CT_F32)
-        hipLaunchKernelGGL((convert_kernel<double, float>), g, b, 0, h->stream, (const double*)in, (float*)out, n);
+    if (tin == CT_F64 && tout == CT_F32 && n % 4 == 0 && (uintptr_t)in % 16 == 0 && (uintptr_t)out % 16 == 0)
+        hipLaunchKernelGGL(convert4_kernel, dim3(grid_for(n / 4)), b, 0, h->stream, (const double2*)in, (float4*)out,
+                           n / 4, active);
+    else if (tin == CT_F64 && tout == CT_F32)
+        hipLaunchKernelGGL((convert_kernel<double, float>), g, b, 0, h->stream, (const double*)in, (float*)out, n,
+                           active);
     else if (tin == CT_F32 && tout == CT_F64)
-        hipLaunchKernelGGL((convert_kernel<float, double>), g, b, 0, h->stream, (const float*)in, (double*)out, n);
-    else if (tin == tout) {
+        hipLaunchKernelGGL((convert_kernel<float, double>), g, b, 0, h->stream, (const float*)in, (double*)out, n,
+                           active);
+    else if (tin == tout && active) {
+        set_last_error("convert: a gated copy needs different types");
+        return CT_STATUS_NOT_SUPPORTED;
+    } else if (tin == tout) {
         CT_HIP(hipMemcpyAsync(out, in, (size_t)n * (tin == CT_F64 ? 8 : 4), hipMemcpyDeviceToDevice, h->stream));
         return 0;
     } else {
