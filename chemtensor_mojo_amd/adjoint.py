@@ -119,6 +119,90 @@ def coupling_gradient(ctx: DeviceContext, mps: MatrixProductState, model: Coupli
     return E, dE
 
 
+def heff_adjoint(L: DenseTensor, R: DenseTensor, W_i: DenseTensor, W_ip1: DenseTensor, theta: DenseTensor,
+                 ctx: DeviceContext):
+    """Gradients of the unnormalised two-site energy <theta|H_eff(W_i, W_ip1)|theta> with respect to the two MPO
+    tensors (ct_heff_adjoint, FP64):
+        gWi[wl,s1,s1',wm] = sum L[wl,a,a'] theta[a,s1,s2,b] W_ip1[wm,s2,s2',wr] R[wr,b,b'] theta[a',s1',s2',b']
+        gWj[wm,s2,s2',wr] = the same network with W_ip1 removed and W_i kept."""
+    Dl, d, _, Dr = theta.shape
+    wL, wM, wR = W_i.shape[0], W_i.shape[3], W_ip1.shape[3]
+    gi = torch.empty(wL * d * d * wM, dtype=torch.float64, device=ctx.device)
+    gj = torch.empty(wM * d * d * wR, dtype=torch.float64, device=ctx.device)
+    call("ct_heff_adjoint", ctx.handle, L.ptr(), W_i.ptr(), W_ip1.ptr(), R.ptr(), theta.ptr(),
+         ctypes.c_void_p(gi.data_ptr()), ctypes.c_void_p(gj.data_ptr()), Dl, d, Dr, wL, wM, wR)
+    return gi.view(wL, d, d, wM), gj.view(wM, d, d, wR)
+
+
+class TwoSiteEnergy(torch.autograd.Function):
+    """E = <theta|H_eff|theta> / <theta|theta> of the two-site local problem
+    (H_eff of apply_two_site_heff, dmrg.mojo:254-374), differentiable in
+    theta, W_i and W_ip1 (FP64 device tensors; L and R are constants):
+
+        dE/dW_i, dE/dW_ip1 = ct_heff_adjoint / <theta|theta>
+        dE/dtheta          = (H_eff + H_eff^T) theta / <theta|theta> - 2 E theta / <theta|theta>
+
+    H_eff^T theta is the matvec with every operand transposed (L[w]^T,
+    W with its two physical indices swapped, R[w]^T), so no symmetry of H is
+    assumed.
+
+    forward(theta [Dl,d,d,Dr], W_i [wL,d,d,wM], W_ip1 [wM,d,d,wR],
+            L [wL,Dl,Dl], R [wR,Dr,Dr], ctx)"""
+
+    @staticmethod
+    def forward(fctx, theta, W_i, W_ip1, L, R, ctx: DeviceContext):
+        if L.requires_grad or R.requires_grad:
+            raise ValueError("TwoSiteEnergy: the environments are constants (no gradient with respect to L or R)")
+        for t in (theta, W_i, W_ip1, L, R):
+            if t.dtype != torch.float64 or t.device.type != "cuda":
+                raise ValueError("TwoSiteEnergy: FP64 device tensors required")
+        Dl, d, _, Dr = theta.shape
+        if L.shape[1:] != (Dl, Dl) or R.shape[1:] != (Dr, Dr) or W_i.shape[0] != L.shape[0] \
+                or W_ip1.shape[3] != R.shape[0] or W_i.shape[3] != W_ip1.shape[0]:
+            raise ValueError("TwoSiteEnergy: shape mismatch")
+        th, Wi, Wj, Lc, Rc = (DenseTensor(t.detach().contiguous()) for t in (theta, W_i, W_ip1, L, R))
+        Hth = _heff(th, Lc, Rc, Wi, Wj, ctx)
+        nrm = float(torch.dot(th.storage, th.storage))
+        E = float(torch.dot(th.storage, Hth)) / nrm
+        fctx.save_for_backward(theta.detach(), W_i.detach(), W_ip1.detach(), L.detach(), R.detach(), Hth)
+        fctx.ctx, fctx.E, fctx.nrm = ctx, E, nrm
+        return torch.tensor(E, dtype=torch.float64, device=theta.device)
+
+    @staticmethod
+    def backward(fctx, gE):
+        theta, W_i, W_ip1, L, R, Hth = fctx.saved_tensors
+        ctx, E, nrm = fctx.ctx, fctx.E, fctx.nrm
+        gth = gWi = gWj = None
+        th = DenseTensor(theta.contiguous())
+        if fctx.needs_input_grad[1] or fctx.needs_input_grad[2]:
+            gi, gj = heff_adjoint(DenseTensor(L.contiguous()), DenseTensor(R.contiguous()),
+                                  DenseTensor(W_i.contiguous()), DenseTensor(W_ip1.contiguous()), th, ctx)
+            gWi = gE * gi / nrm if fctx.needs_input_grad[1] else None
+            gWj = gE * gj / nrm if fctx.needs_input_grad[2] else None
+        if fctx.needs_input_grad[0]:
+            # H^T theta: the same matvec on transposed operands
+            Lt = DenseTensor(L.transpose(1, 2).contiguous())
+            Rt = DenseTensor(R.transpose(1, 2).contiguous())
+            Wit = DenseTensor(W_i.transpose(1, 2).contiguous())
+            Wjt = DenseTensor(W_ip1.transpose(1, 2).contiguous())
+            HTth = _heff(th, Lt, Rt, Wit, Wjt, ctx)
+            g = (Hth + HTth - 2.0 * E * th.storage) / nrm
+            gth = gE * g.view(theta.shape)
+        return gth, gWi, gWj, None, None, None
+
+
+def _heff(th: DenseTensor, L: DenseTensor, R: DenseTensor, Wi: DenseTensor, Wj: DenseTensor,
+          ctx: DeviceContext) -> torch.Tensor:
+    from .dmrg import apply_two_site_heff
+    return apply_two_site_heff(th, L, R, MPOSite(Wi), MPOSite(Wj), ctx, prec=_lib.PREC_FP64).storage
+
+
+def two_site_energy(theta: torch.Tensor, W_i: torch.Tensor, W_ip1: torch.Tensor, L: torch.Tensor, R: torch.Tensor,
+                    ctx: DeviceContext) -> torch.Tensor:
+    """Differentiable two-site local energy (see TwoSiteEnergy)."""
+    return TwoSiteEnergy.apply(theta, W_i, W_ip1, L, R, ctx)
+
+
 class DMRGEnergy(torch.autograd.Function):
     """E(J) = ground-state energy of H(J) by two-site DMRG (forward), with
     dE/dJ from the adjoint environment contractions (backward).
@@ -148,5 +232,5 @@ def dmrg_energy(J: torch.Tensor, solver, model: CouplingModel, ctx: DeviceContex
     return DMRGEnergy.apply(J, solver, model, ctx)
 
 
-__all__ = ["env_adjoint", "energy_and_mpo_gradient", "expectation", "CouplingModel", "random_bond_heisenberg_model",
+__all__ = ["env_adjoint", "heff_adjoint", "TwoSiteEnergy", "two_site_energy", "energy_and_mpo_gradient", "expectation", "CouplingModel", "random_bond_heisenberg_model",
            "coupling_gradient", "DMRGEnergy", "dmrg_energy"]

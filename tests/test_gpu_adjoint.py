@@ -50,6 +50,34 @@ def test_heff_adjoint_vs_einsum(ctx):
     assert np.allclose(gj.cpu().numpy().reshape(refj.shape), refj, rtol=1e-11, atol=1e-11 * np.abs(refj).max())
 
 
+def test_two_site_energy_autograd_vs_cpu_einsum(ctx):
+    """TwoSiteEnergy: E and dE/d(theta, W_i, W_ip1) against torch autograd of the
+    same network written as an einsum on the CPU (non-symmetric L, R, W, so the
+    H^T theta term is exercised)."""
+    rng = np.random.default_rng(11)
+    Dl, d, Dr, wL, wM, wR = 5, 2, 6, 3, 4, 2
+    L = rng.standard_normal((wL, Dl, Dl))
+    R = rng.standard_normal((wR, Dr, Dr))
+    Wi = rng.standard_normal((wL, d, d, wM))
+    Wj = rng.standard_normal((wM, d, d, wR))
+    th = rng.standard_normal((Dl, d, d, Dr))
+    cpu = [torch.tensor(x, requires_grad=True) for x in (th, Wi, Wj)]
+    Lc, Rc = torch.tensor(L), torch.tensor(R)
+    num = torch.einsum("lab,asrc,lsum,mrvn,ncd,buvd->", Lc, cpu[0], cpu[1], cpu[2], Rc, cpu[0])
+    E_ref = num / torch.sum(cpu[0] * cpu[0])
+    E_ref.backward()
+    gpu = [torch.tensor(x, device="cuda", requires_grad=True) for x in (th, Wi, Wj)]
+    E = adj.two_site_energy(gpu[0], gpu[1], gpu[2], torch.tensor(L, device="cuda"), torch.tensor(R, device="cuda"), ctx)
+    E.backward()
+    assert abs(E.item() - E_ref.item()) <= 1e-12 * abs(E_ref.item())
+    for g, c in zip(gpu, cpu):
+        gg, gc = g.grad.cpu().numpy(), c.grad.numpy()
+        assert np.allclose(gg, gc, rtol=1e-11, atol=1e-11 * np.abs(gc).max())
+    with pytest.raises(ValueError):
+        adj.two_site_energy(gpu[0], gpu[1], gpu[2], torch.tensor(L, device="cuda", requires_grad=True),
+                            torch.tensor(R, device="cuda"), ctx)
+
+
 def statevector(mps_np):
     psi = mps_np[0]
     for A in mps_np[1:]:
