@@ -655,6 +655,35 @@ def study_eigensolver(ctx, args):
     print(json.dumps({"study": "eigensolver", "by_n": bench_eigensolver(ctx, sizes=(2048,))}), flush=True)
 
 
+def bench_sweeps(ctx, args):
+    """Fixed-D sweeps in FP64 and the refined FP32 / 3xBF16 forms on the two chains (see the module docstring)."""
+    import chemtensor_mojo_amd as ct
+    sweep = {}
+    for key, model, L_sites, nsw in (("j1j2_mg", "j1j2", args.sweep_sites, 2),
+                                     ("heisenberg_afm", "heisenberg", args.afm_sites, 2)):
+        runs = {name: bench_sweep(ctx, L_sites, args.sweep_D, prec, refine, model=model, sweeps=nsw)
+                for name, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
+                                           ("bf16x3_refined", ct.PREC_BF16X3, 2))}
+        E64 = runs["fp64"]["energies"][-1]
+        desc = ("J1-J2 chain at the Majumdar-Ghosh point (J2=0.5, w=8): after the first sweep every local "
+                "problem is converged, so the sweep times the truncation and a one-vector Lanczos"
+                if model == "j1j2" else
+                "antiferromagnetic Heisenberg chain (w=5), far from converged at this bond dimension: every "
+                "local Lanczos runs its full Krylov space")
+        sweep[key] = {
+            "workload": (f"{desc}; L={L_sites}, fixed D={args.sweep_D} (eps_trunc=0), PCG32 random start at "
+                         f"full D, 20 Krylov vectors, exact tridiagonal solves, Gram-matrix FP64 truncation, "
+                         f"{nsw} sweeps"),
+            **runs,
+            "dE_rel_bf16x3_vs_fp64": abs(runs["bf16x3_refined"]["energies"][-1] - E64) / abs(E64),
+            "dE_rel_fp32_vs_fp64": abs(runs["fp32_refined"]["energies"][-1] - E64) / abs(E64),
+            "speedup_sweep_bf16x3": [a / b for a, b in zip(runs["fp64"]["sweep_s"],
+                                                           runs["bf16x3_refined"]["sweep_s"])],
+            "speedup_sweep_fp32": [a / b for a, b in zip(runs["fp64"]["sweep_s"], runs["fp32_refined"]["sweep_s"])],
+        }
+    return sweep
+
+
 def cpu_baseline(D, w, budget_s):
     from oracle import oracle as ora
     ora.build()
@@ -715,6 +744,8 @@ def main():
     ap.add_argument("--sweep-D", type=int, default=1024)
     ap.add_argument("--afm-sites", type=int, default=40)
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="run only the fixed-D sweeps and print them as one JSON line (for kernel traces of the sweeps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the eigensolver and matvec-grid sections")
     ap.add_argument("--no-configs", action="store_true",
@@ -742,35 +773,15 @@ def main():
          "qc16_matvec": study_qc16_matvec}[args.study](ctx, args)
         ctx.close()
         return
+    if args.sweep_only:
+        print(json.dumps({"sweep": bench_sweeps(ctx, args)}), flush=True)
+        ctx.close()
+        return
     mv, F = bench_matvec(ctx, D, w, args.steps, args.warmup)
     gm, Fg = bench_dominant_gemm(ctx, D, w, args.steps, args.warmup)
     eig = None if args.no_extras else bench_eigensolver(ctx)
     mgrid = None if args.no_extras else bench_matvec_grid(ctx, max(5, args.steps // 2), args.warmup)
-    sweep = None
-    if not args.no_sweep:
-        sweep = {}
-        for key, model, L_sites, nsw in (("j1j2_mg", "j1j2", args.sweep_sites, 2),
-                                         ("heisenberg_afm", "heisenberg", args.afm_sites, 2)):
-            runs = {name: bench_sweep(ctx, L_sites, args.sweep_D, prec, refine, model=model, sweeps=nsw)
-                    for name, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
-                                               ("bf16x3_refined", ct.PREC_BF16X3, 2))}
-            E64 = runs["fp64"]["energies"][-1]
-            desc = ("J1-J2 chain at the Majumdar-Ghosh point (J2=0.5, w=8): after the first sweep every local "
-                    "problem is converged, so the sweep times the truncation and a one-vector Lanczos"
-                    if model == "j1j2" else
-                    "antiferromagnetic Heisenberg chain (w=5), far from converged at this bond dimension: every "
-                    "local Lanczos runs its full Krylov space")
-            sweep[key] = {
-                "workload": (f"{desc}; L={L_sites}, fixed D={args.sweep_D} (eps_trunc=0), PCG32 random start at "
-                             f"full D, 20 Krylov vectors, exact tridiagonal solves, Gram-matrix FP64 truncation, "
-                             f"{nsw} sweeps"),
-                **runs,
-                "dE_rel_bf16x3_vs_fp64": abs(runs["bf16x3_refined"]["energies"][-1] - E64) / abs(E64),
-                "dE_rel_fp32_vs_fp64": abs(runs["fp32_refined"]["energies"][-1] - E64) / abs(E64),
-                "speedup_sweep_bf16x3": [a / b for a, b in zip(runs["fp64"]["sweep_s"],
-                                                               runs["bf16x3_refined"]["sweep_s"])],
-                "speedup_sweep_fp32": [a / b for a, b in zip(runs["fp64"]["sweep_s"], runs["fp32_refined"]["sweep_s"])],
-            }
+    sweep = None if args.no_sweep else bench_sweeps(ctx, args)
     configs = None
     if not args.no_configs:
         _, _, c2 = config2_sweeps(ctx, ((ct.SVD_GRAM, "fp64_gram"),))
