@@ -35,14 +35,21 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
     return (uint16_t)(u >> 16);
 }
 
+template <typename T, int N>
+struct alignas(sizeof(T) * N) VecN {
+    T v[N];
+};
+
 // OUT: 0 = double, 1 = float, 2 = split BF16 hi/lo planes (uint16).
 // KM: compile-time size (16, 20, 32 or 64) of the zero-padded KM x KM coefficient
 // block `coef` (row k = input, column n = output).  Inputs k >= K re-read
 // input K-1 and meet zero coefficients, so the unrolled loops carry no
 // per-element guards; every coefficient address is a compile-time offset
 // from a kernel argument, so the FMAs take them from SGPRs loaded through
-// the scalar cache.
-template <typename TX, typename TACC, int OUT, int KM, bool FULL>
+// the scalar cache.  VQ: consecutive pixels q per thread, loaded and stored
+// as one VQ-element vector; the launch guarantees Q % VQ == 0 and VQ-aligned
+// bases and strides.
+template <typename TX, typename TACC, int OUT, int KM, bool FULL, int VQ>
 __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void* __restrict__ Yv,
                                                   const TACC* __restrict__ coef, MixArgs a) {
     if (a.active && *a.active == 0) return;
@@ -64,40 +71,61 @@ __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void
         }
     }
     __syncthreads();
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t q = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * VQ;
     if (q >= a.Q) return;
     for (int64_t p = blockIdx.y; p < a.P; p += gridDim.y) {
         // Keeps the coefficient loads inside the loop: hoisting all KM*KM of
         // them out of it would spill.
         asm volatile("" ::: "memory");
         const TX* xp = X + p * a.sXp + q;
-        TACC x[KM];
+        TACC x[KM][VQ];
 #pragma unroll
-        for (int k = 0; k < KM; ++k) x[k] = (TACC)xp[in_off[k]];
-        TACC acc[KM];
+        for (int k = 0; k < KM; ++k) {
+            const VecN<TX, VQ> t = *(const VecN<TX, VQ>*)(xp + in_off[k]);
 #pragma unroll
-        for (int n = 0; n < KM; ++n) acc[n] = 0;
+            for (int j = 0; j < VQ; ++j) x[k][j] = (TACC)t.v[j];
+        }
+        TACC acc[KM][VQ];
+#pragma unroll
+        for (int n = 0; n < KM; ++n)
+#pragma unroll
+            for (int j = 0; j < VQ; ++j) acc[n][j] = 0;
 #pragma unroll
         for (int k = 0; k < KM; ++k)
 #pragma unroll
-            for (int n = 0; n < KM; ++n) acc[n] += coef[k * KM + n] * x[k];
+            for (int n = 0; n < KM; ++n) {
+                const TACC cf = coef[k * KM + n];
+#pragma unroll
+                for (int j = 0; j < VQ; ++j) acc[n][j] += cf * x[k][j];
+            }
         // FULL (NO == KM): no guard, so the loop stays unrolled
 #pragma unroll
         for (int n = 0; n < KM; ++n) {
             if (!FULL && n >= NO) break;
             if (OUT == 0) {
-                ((double*)Yv)[out_off[n] + p * a.sYp + q] = (double)acc[n];
+                VecN<double, VQ> o;
+#pragma unroll
+                for (int j = 0; j < VQ; ++j) o.v[j] = (double)acc[n][j];
+                *(VecN<double, VQ>*)((double*)Yv + out_off[n] + p * a.sYp + q) = o;
             } else if (OUT == 1) {
-                ((float*)Yv)[out_off[n] + p * a.sYp + q] = (float)acc[n];
+                VecN<float, VQ> o;
+#pragma unroll
+                for (int j = 0; j < VQ; ++j) o.v[j] = (float)acc[n][j];
+                *(VecN<float, VQ>*)((float*)Yv + out_off[n] + p * a.sYp + q) = o;
             } else {
                 // split rows: hi at 2*(k & ~31) + (k & 31), lo 32 elements later
+                // (q .. q+VQ-1 stay inside one 32-wide block: Q % 32 == 0 when VQ > 1)
                 const int64_t k = out_k[n] + q;
                 const int64_t o = out_off[n] + p * a.sYp + 2 * (k & ~(int64_t)31) + (k & 31);
-                const float f = (float)acc[n];
-                const uint16_t hb = f2bf(f);
-                const float hf = __uint_as_float(((uint32_t)hb) << 16);
-                ((uint16_t*)Yv)[o] = hb;
-                ((uint16_t*)Yv)[o + 32] = f2bf(f - hf);
+                VecN<uint16_t, VQ> hv, lv;
+#pragma unroll
+                for (int j = 0; j < VQ; ++j) {
+                    const float f = (float)acc[n][j];
+                    hv.v[j] = f2bf(f);
+                    lv.v[j] = f2bf(f - __uint_as_float(((uint32_t)hv.v[j]) << 16));
+                }
+                *(VecN<uint16_t, VQ>*)((uint16_t*)Yv + o) = hv;
+                *(VecN<uint16_t, VQ>*)((uint16_t*)Yv + o + 32) = lv;
             }
         }
     }
@@ -233,6 +261,39 @@ static int mix_km(int K, int NO) {
 }
 static int mix_km(const MixArgs& a) { return mix_km(a.CIN * a.EIN, a.COUT * a.EOUT); }
 
+template <typename TX, typename TACC, int OUT, int VQ>
+int launch_mix_vq(Handle* h, const TX* X, void* Y, const TACC* c, const MixArgs& a, int NO, int KM) {
+    const int64_t threads = a.Q / VQ;
+    dim3 grid((unsigned)((threads + 255) / 256), (unsigned)(a.P < 65535 ? a.P : 65535)), block(256);
+    // Every output channel real (NO == KM): the unguarded store loop keeps acc[]
+    // in registers (a guarded loop is rolled and, in FP64 or with VQ > 1, puts
+    // acc[] in scratch).  FP32 with VQ = 1 keeps the guarded loop in registers
+    // at higher occupancy and measures faster.
+    const bool full = NO == KM && (sizeof(TACC) == 8 || VQ > 1);
+    if (full) {
+        if (KM == 16)
+            hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16, true, VQ>), grid, block, 0, h->stream, X, Y, c, a);
+        else if (KM == 20)
+            hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 20, true, VQ>), grid, block, 0, h->stream, X, Y, c, a);
+        else if (KM == 32)
+            hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32, true, VQ>), grid, block, 0, h->stream, X, Y, c, a);
+        else if constexpr (VQ == 1)
+            hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 64, true, 1>), grid, block, 0, h->stream, X, Y, c, a);
+        return 0;
+    }
+    if constexpr (VQ == 1) {
+        if (KM == 16)
+            hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16, false, 1>), grid, block, 0, h->stream, X, Y, c, a);
+        else if (KM == 20)
+            hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 20, false, 1>), grid, block, 0, h->stream, X, Y, c, a);
+        else if (KM == 32)
+            hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32, false, 1>), grid, block, 0, h->stream, X, Y, c, a);
+        else
+            hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 64, false, 1>), grid, block, 0, h->stream, X, Y, c, a);
+    }
+    return 0;
+}
+
 // Padded coefficient block (workspace, unless `padded` already holds it) in
 // the accumulation type, then the mixing launch.
 template <typename TX, typename TACC, int OUT>
@@ -246,32 +307,20 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
                            (TACC*)w, K, NO, KM);
         c = (const TACC*)w;
     }
-    dim3 grid((unsigned)((a.Q + 255) / 256), (unsigned)(a.P < 65535 ? a.P : 65535)), block(256);
-    // FP64 with every output channel real: the unguarded store loop keeps acc[] in
-    // registers (the guarded loop is rolled and puts acc[] in scratch).  For FP32 the
-    // guarded loop stays in registers at higher occupancy and measures faster.
-    if constexpr (sizeof(TACC) == 8) {
-        if (NO == KM) {
-            if (KM == 16)
-                hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16, true>), grid, block, 0, h->stream, X, Y, c, a);
-            else if (KM == 20)
-                hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 20, true>), grid, block, 0, h->stream, X, Y, c, a);
-            else if (KM == 32)
-                hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32, true>), grid, block, 0, h->stream, X, Y, c, a);
-            else
-                hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 64, true>), grid, block, 0, h->stream, X, Y, c, a);
-            return 0;
-        }
-    }
-    if (KM == 16)
-        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 16, false>), grid, block, 0, h->stream, X, Y, c, a);
-    else if (KM == 20)
-        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 20, false>), grid, block, 0, h->stream, X, Y, c, a);
-    else if (KM == 32)
-        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 32, false>), grid, block, 0, h->stream, X, Y, c, a);
-    else
-        hipLaunchKernelGGL((mix_kernel<TX, TACC, OUT, 64, false>), grid, block, 0, h->stream, X, Y, c, a);
-    return 0;
+    // FP64 inputs: two consecutive pixels per thread (16-byte accesses) when
+    // every output channel is real and every base, stride and Q allow it
+    // (D = 1024, w = 8: 133 -> 127 us).  FP32 inputs keep one pixel per thread:
+    // two or four per thread measured the same or slower (the split-row output
+    // 63 -> 79 / 98 us), as the register use of KM * VQ inputs and sums cuts
+    // the occupancy from 7 waves per SIMD to 3 or 1.
+    constexpr int VQV = sizeof(TX) == 8 ? 2 : 1;
+    const auto div = [](int64_t v) { return v % VQV == 0; };
+    const bool vec = VQV > 1 && KM <= 32 && NO == KM && div(a.Q) && div(a.sXc) && div(a.sXp) && div(a.sXe) && div(a.sYc) && div(a.sYp) &&
+                     div(a.sYe) && (uintptr_t)X % 16 == 0 &&
+                     (OUT == 2 ? (a.Q % 32 == 0 && a.sYc % 32 == 0 && (uintptr_t)Y % (size_t)(VQV * 2) == 0)
+                               : (uintptr_t)Y % (size_t)(VQV * (OUT == 0 ? 8 : 4)) == 0);
+    if (vec) return launch_mix_vq<TX, TACC, OUT, VQV>(h, X, Y, c, a, NO, KM);
+    return launch_mix_vq<TX, TACC, OUT, 1>(h, X, Y, c, a, NO, KM);
 }
 
 // ------------------------------------------------ large MPO bond dimension

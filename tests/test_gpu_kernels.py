@@ -49,6 +49,9 @@ DOT_CASES = [
     # k-contiguous A (and B), ragged K through the row-slice tile loader
     ((1024, 4096), (4096, 2048), 1, False, True),
     ((2048, 3000), (1536, 3000), 1, False, False),
+    # whole 128x128 tiles with K a multiple of 16 but not of 32: the FP32 path's
+    # unguarded tiles fall back from the 32-wide K step to 16
+    ((256, 1040), (1040, 384), 1, False, True),
 ]
 
 
