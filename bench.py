@@ -17,7 +17,8 @@ events on the stream it runs on; fixed-D D=1024 DMRG sweeps in FP64 and in
 the refined FP32 / 3xBF16 forms on two chains - one whose local problems are
 converged after the first sweep (Majumdar-Ghosh point) and one whose Lanczos
 runs its whole Krylov space (antiferromagnetic Heisenberg); the symmetric
-eigensolver of the truncation; the matvec grid over D and w; whole DMRG runs
+eigensolver of the truncation; the matvec grid over D and w; the environment
+updates and the theta build at D=1024; whole DMRG runs
 of two benchmark configurations with per-stage times (XXZ antiferromagnet L=64,
 chi <= 256, FP64; J1-J2 L=100, chi <= 1024, all three forms); and the FP64 CPU
 oracle timed on the host cores.
@@ -156,6 +157,46 @@ def bench_matvec_mpo(ctx, D, Wi, Wj, steps, warmup, seed=7):
     del tL, tR, tth
     torch.cuda.empty_cache()
     return res, F
+
+
+def env_flops(D, d, w):
+    """Algorithmic FLOPs of update_left/right_environment with all bonds D (environments.mojo:99-201, 309-417):
+    2 wL Dl Dl' d Dr + 2 Dl' Dr wL d^2 wR + 2 Dr wR Dl' d Dr' = 4 w d D^3 + 2 w^2 d^2 D^2."""
+    return 4.0 * w * d * D ** 3 + 2.0 * w * w * d * d * D ** 2
+
+
+def bench_env_theta(ctx, steps, warmup, D=1024, ws=(5, 8)):
+    """Left / right environment updates (FP64, FP32, 3xBF16 contractions on FP64 storage) and the two-site theta
+    build (FP64) at bond dimension D, seeded random site tensor and environment, XXZ (w=5) / J1-J2 (w=8) bulk
+    MPO; mean of `steps` calls timed with HIP events."""
+    import torch
+    import chemtensor_mojo_amd as ct
+    from chemtensor_mojo_amd import hamiltonians as ham
+    rng = np.random.default_rng(11)
+    dev = lambda a: ct.create_dense_tensor_from_data(ctx, a, a.shape)
+    A = dev(rng.standard_normal((D, 2, D)) / np.sqrt(2 * D))
+    out = {}
+    for w in ws:
+        W = ham.j1j2_tensors(6, 1.0, 0.5)[2] if w == 8 else ham.heisenberg_xxz_tensors(6, 1.0, 1.0, 0.0)[2]
+        E = rng.standard_normal((w, D, D)) / np.sqrt(D)
+        E = dev(0.5 * (E + E.transpose(0, 2, 1)))
+        site, mpo = ct.MPSSite(A), ct.MPOSite(dev(W))
+        F = env_flops(D, 2, w)
+        row = {"gflop": round(F / 1e9, 3)}
+        for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3)):
+            tl, _ = time_events(lambda: ct.update_left_environment(E, site, mpo, ctx, prec=prec), steps, warmup)
+            tr, _ = time_events(lambda: ct.update_right_environment(E, site, mpo, ctx, prec=prec), steps, warmup)
+            row[name] = {"left_ms": round(tl * 1e3, 4), "left_tflops": round(F / tl / 1e12, 2),
+                         "right_ms": round(tr * 1e3, 4), "right_tflops": round(F / tr / 1e12, 2)}
+            log(f"env update w={w} D={D} {name}: left {tl * 1e3:.3f} ms ({F / tl / 1e12:.1f} TFLOP/s), "
+                f"right {tr * 1e3:.3f} ms ({F / tr / 1e12:.1f} TFLOP/s)")
+        out[f"w{w}"] = row
+    Ft = 2.0 * (2 * D) * D * (2 * D)
+    tt, _ = time_events(lambda: ct.build_two_site_theta(ct.MPSSite(A), ct.MPSSite(A), ctx), steps, warmup)
+    out["theta_build_fp64"] = {"gflop": round(Ft / 1e9, 3), "ms": round(tt * 1e3, 4), "tflops": round(Ft / tt / 1e12, 2)}
+    log(f"theta build D={D} fp64: {tt * 1e3:.3f} ms ({Ft / tt / 1e12:.1f} TFLOP/s)")
+    torch.cuda.empty_cache()
+    return out
 
 
 def bench_matvec_grid(ctx, steps, warmup, Ds=(256, 512, 1024), ws=(5, 8)):
@@ -781,6 +822,7 @@ def main():
     gm, Fg = bench_dominant_gemm(ctx, D, w, args.steps, args.warmup)
     eig = None if args.no_extras else bench_eigensolver(ctx)
     mgrid = None if args.no_extras else bench_matvec_grid(ctx, max(5, args.steps // 2), args.warmup)
+    envs = None if args.no_extras else bench_env_theta(ctx, max(5, args.steps // 2), args.warmup)
     sweep = None if args.no_sweep else bench_sweeps(ctx, args)
     configs = None
     if not args.no_configs:
@@ -844,6 +886,7 @@ def main():
         "configs": configs,
         "eigensolver": {"workload": bench_eigensolver.__doc__.replace("\n", " ").replace("    ", " "), "by_n": eig},
         "matvec_grid": {"workload": bench_matvec_grid.__doc__.replace("\n", " ").replace("    ", " "), "by_w_D": mgrid},
+        "env_update_theta": {"workload": " ".join(bench_env_theta.__doc__.split()), "by_w": envs},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
