@@ -7,8 +7,8 @@
 // uses v_mfma_f32_16x16x32_bf16 with a k-contiguous LDS layout.
 //
 // Block tile 128x128 (64x64 for small problems), K-tile 16 (FP64, guarded
-// FP32 tiles) or 32 (unguarded FP32 tiles), 256 threads = 4 waves in a 2x2 grid, each wave owning a 64x64
-// (32x32) sub-tile = 4x4 (2x2) MFMA tiles.  Operands may be m/n-contiguous or
+// FP32 tiles) or 32 (unguarded FP32 tiles), 256 threads = 4 waves in a 2x2
+// grid, each wave owning a 64x64 (32x32) sub-tile = 4x4 (2x2) MFMA tiles.  Operands may be m/n-contiguous or
 // k-contiguous (the transposes the reference performs with separate permute
 // kernels, dense_tensor.mojo:1249-1257, happen here inside the tile load).
 // An m/n-contiguous operand is staged in LDS k-major (sX[k][x]), a
