@@ -398,11 +398,14 @@ int tensordot(Handle* h, int prec, int dtype, const void* A, int rA, const int64
 // ----------------------------------------------------------------- adjoint
 // gWi[wl,s1,s1',wm] = sum L[wl,a,a'] th[a,s1,s2,b] Wj[wm,s2,s2',wr] R[wr,b,b'] th[a',s1',s2',b']
 // gWj[wm,s2,s2',wr] = sum L th Wi R th
-// Computed as
-//   T1[wl][a'][s1 s2][b]  = L^T th                                (GEMM, batched over wl)
-//   X [wr][a' s1' s2'][b] = th[(a' s1' s2')][b'] R[wr][b][b']^T   (GEMM, batched over wr)
-//   G [wl,s1,s2 ; wr,s1',s2'] = sum_{a',b} T1[wl][a'][s1 s2][b] X[wr][a' s1' s2'][b]
-// followed by the tiny contractions of G with Wj (for gWi) or Wi (for gWj).
+// Computed as (e = (s1 s2), eo = (s1' s2'), E = d*d)
+//   T1[wl][e][a'][b]  = sum_a L[wl][a][a'] th[a][e][b]             (E GEMMs, batched over wl)
+//   X [wr][eo][a'][b] = sum_b' th[a'][eo][b'] R[wr][b][b']          (E GEMMs, batched over wr)
+//   G [wl,e ; wr,eo]  = sum_{(a',b)} T1[(wl,e)][(a',b)] X[(wr,eo)][(a',b)]
+// The (e, a', b) order makes (a', b) one contiguous index of length Dlb*Dr, so
+// G is a single k-contiguous GEMM of M = wL*E, N = wR*E, K = Dlb*Dr, split over
+// K across the CUs: T1 and X are each read once.  Then the tiny contractions of
+// G with Wj (for gWi) or Wi (for gWj).
 namespace {
 __global__ void adj_contract_kernel(const double* __restrict__ G, const double* __restrict__ Wo, int wL, int wM,
                                     int wR, int d, int which, double* __restrict__ out) {
@@ -436,24 +439,20 @@ __global__ void adj_contract_kernel(const double* __restrict__ G, const double* 
     }
 }
 
-// G[(wl, e) ; (wr, eo)] = sum_{a', b} T1[wl][a'][e][b] * X[wr][a'][eo][b], e/eo < E
-// (E = d*d for the two-site network, d for one site).  One block per output
-// element, reduction over (a', b); b is contiguous in both operands.
-__global__ __launch_bounds__(256) void adj_gram_kernel(const double* __restrict__ T1, const double* __restrict__ X,
-                                                       int64_t P, int64_t Q, int E, int wR, double* __restrict__ G) {
-    const int NO = wR * E;
-    const int o = blockIdx.x;  // over wL*E*NO
-    const int k = o / NO, c = o - k * NO;
-    const int wl = k / E, e = k % E;
-    const int wr = c / E, eo = c % E;
-    double s = 0.0;
-    const int64_t n = P * Q;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const int64_t a = i / Q, b = i - a * Q;
-        s += T1[((wl * P + a) * E + e) * Q + b] * X[(((int64_t)wr * P + a) * E + eo) * Q + b];
+// G[(wl, e) ; (wr, eo)] = sum_{(a, b)} T1[wl][e][(a, b)] * X[wr][eo][(a, b)]:
+// one k-contiguous GEMM, C = A B^T, with K = P*Q (both operands stored [w][e][P*Q]).
+int adj_gram(Handle* h, const double* T1, const double* X, int64_t P, int64_t Q, int64_t E, int64_t wL, int64_t wR,
+             double* G) {
+    if (P * Q > INT32_MAX || wL * E > INT32_MAX || wR * E > INT32_MAX) {
+        set_last_error("adjoint: Dl*Dr or w*E exceeds the 32-bit GEMM extents");
+        return CT_STATUS_NOT_SUPPORTED;
     }
-    s = block_sum256(s);
-    if (threadIdx.x == 0) G[o] = s;
+    GemmDesc q;
+    q.M = (int)(wL * E); q.N = (int)(wR * E); q.K = (int)(P * Q); q.batch = 1;
+    q.a_kc = 1; q.sAm = P * Q; q.sAk = 1;
+    q.b_kc = 1; q.sBn = P * Q; q.sBk = 1;
+    q.ldc = wR * E;
+    return gemm(h, CT_PREC_FP64, CT_F64, T1, CT_F64, X, CT_F64, G, q);
 }
 }  // namespace
 
@@ -466,25 +465,23 @@ int heff_adjoint(Handle* h, const double* L, const double* Wi, const double* Wj,
     CT_CHECK(ws_get(h, WS_TMP, (size_t)nT1 * 8, &t1));
     CT_CHECK(ws_get(h, WS_TMP2, (size_t)nX * 8, &x));
     CT_CHECK(ws_get(h, WS_TMP3, (size_t)nG * 8, &g));
-    {
+    for (int64_t e = 0; e < d2; ++e) {  // T1[wl][e][a'][b] = sum_a L[wl][a][a'] th[a][e][b]
         GemmDesc q;
-        q.M = (int)D.Dlb; q.N = (int)(d2 * D.Dr); q.K = (int)D.Dl; q.batch = (int)D.wL;
+        q.M = (int)D.Dlb; q.N = (int)D.Dr; q.K = (int)D.Dl; q.batch = (int)D.wL;
         q.a_kc = 0; q.sAm = 1; q.sAk = D.Dlb; q.bsA = D.Dl * D.Dlb;
         q.b_kc = 0; q.sBk = d2 * D.Dr; q.sBn = 1; q.bsB = 0;
-        q.ldc = d2 * D.Dr; q.bsC = D.Dlb * d2 * D.Dr;
-        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, L, CT_F64, theta, CT_F64, t1, q));
+        q.ldc = D.Dr; q.bsC = d2 * D.Dlb * D.Dr;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, L, CT_F64, theta + e * D.Dr, CT_F64, (double*)t1 + e * D.Dlb * D.Dr, q));
     }
-    {   // X[wr][(a' s1o s2o)][b] = sum_{b'} th[(a' s1o s2o)][b'] R[wr][b][b']
+    for (int64_t e = 0; e < d2; ++e) {  // X[wr][eo][a'][b] = sum_{b'} th[a'][eo][b'] R[wr][b][b']
         GemmDesc q;
-        q.M = (int)(D.Dlb * d2); q.N = (int)D.Dr; q.K = (int)D.Drb; q.batch = (int)D.wR;
-        q.a_kc = 1; q.sAm = D.Drb; q.sAk = 1; q.bsA = 0;
+        q.M = (int)D.Dlb; q.N = (int)D.Dr; q.K = (int)D.Drb; q.batch = (int)D.wR;
+        q.a_kc = 1; q.sAm = d2 * D.Drb; q.sAk = 1; q.bsA = 0;
         q.b_kc = 1; q.sBn = D.Drb; q.sBk = 1; q.bsB = D.Dr * D.Drb;
-        q.ldc = D.Dr; q.bsC = D.Dlb * d2 * D.Dr;
-        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, theta, CT_F64, R, CT_F64, x, q));
+        q.ldc = D.Dr; q.bsC = d2 * D.Dlb * D.Dr;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, theta + e * D.Drb, CT_F64, R, CT_F64, (double*)x + e * D.Dlb * D.Dr, q));
     }
-    hipLaunchKernelGGL(adj_gram_kernel, dim3((unsigned)nG), dim3(256), 0, h->stream, (const double*)t1,
-                       (const double*)x, D.Dlb, D.Dr, (int)d2, (int)D.wR, (double*)g);
-    CT_HIP(hipGetLastError());
+    CT_CHECK(adj_gram(h, (const double*)t1, (const double*)x, D.Dlb, D.Dr, d2, D.wL, D.wR, (double*)g));
     if (gWi) {
         const int n = (int)(D.wL * d2 * D.wM);
         hipLaunchKernelGGL(adj_contract_kernel, dim3((n + 255) / 256), dim3(256), 0, h->stream, (const double*)g, Wj,
@@ -503,36 +500,35 @@ int heff_adjoint(Handle* h, const double* L, const double* Wi, const double* Wj,
 // Single-site environment adjoint:
 //   gW[wl,s,s',wr] = sum L[wl,a,a'] A[a,s,b] A[a',s',b'] R[wr,b,b']
 // the derivative of <psi|H|psi> = L_i . A W A . R_{i+1} with respect to W.
-//   T1[wl][a'][s][b] = sum_a L[wl][a][a'] A[a][(s b)]        (GEMM batched over wl)
-//   Y [wr][a'][s'][b] = sum_b' A[(a' s')][b'] R[wr][b][b']   (GEMM batched over wr)
-//   gW = sum_{a',b} T1 Y                                       (adj_gram_kernel, E = d)
+//   T1[wl][s][a'][b]  = sum_a L[wl][a][a'] A[a][s][b]       (d GEMMs batched over wl)
+//   Y [wr][s'][a'][b] = sum_b' A[a'][s'][b'] R[wr][b][b']   (d GEMMs batched over wr)
+//   G [wl,s ; wr,s']  = sum_{(a',b)} T1 Y                   (adj_gram, E = d)
+// and G[wl][s][wr][s'] permuted to gW[wl][s][s'][wr].
 int env_adjoint(Handle* h, const double* L, const double* A, const double* R, double* gW, int64_t Dl, int64_t d,
                 int64_t Dr, int64_t wL, int64_t wR) {
     const int64_t nT1 = wL * Dl * d * Dr, nY = wR * Dl * d * Dr, nG = wL * d * d * wR;
     void *t1, *y;
     CT_CHECK(ws_get(h, WS_TMP, (size_t)nT1 * 8, &t1));
     CT_CHECK(ws_get(h, WS_TMP2, (size_t)nY * 8, &y));
-    {
+    for (int64_t s = 0; s < d; ++s) {
         GemmDesc q;
-        q.M = (int)Dl; q.N = (int)(d * Dr); q.K = (int)Dl; q.batch = (int)wL;
+        q.M = (int)Dl; q.N = (int)Dr; q.K = (int)Dl; q.batch = (int)wL;
         q.a_kc = 0; q.sAm = 1; q.sAk = Dl; q.bsA = Dl * Dl;
         q.b_kc = 0; q.sBk = d * Dr; q.sBn = 1; q.bsB = 0;
-        q.ldc = d * Dr; q.bsC = Dl * d * Dr;
-        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, L, CT_F64, A, CT_F64, t1, q));
+        q.ldc = Dr; q.bsC = d * Dl * Dr;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, L, CT_F64, A + s * Dr, CT_F64, (double*)t1 + s * Dl * Dr, q));
     }
-    {
+    for (int64_t s = 0; s < d; ++s) {
         GemmDesc q;
-        q.M = (int)(Dl * d); q.N = (int)Dr; q.K = (int)Dr; q.batch = (int)wR;
-        q.a_kc = 1; q.sAm = Dr; q.sAk = 1; q.bsA = 0;
+        q.M = (int)Dl; q.N = (int)Dr; q.K = (int)Dr; q.batch = (int)wR;
+        q.a_kc = 1; q.sAm = d * Dr; q.sAk = 1; q.bsA = 0;
         q.b_kc = 1; q.sBn = Dr; q.sBk = 1; q.bsB = Dr * Dr;
-        q.ldc = Dr; q.bsC = Dl * d * Dr;
-        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, A, CT_F64, R, CT_F64, y, q));
+        q.ldc = Dr; q.bsC = d * Dl * Dr;
+        CT_CHECK(gemm(h, CT_PREC_FP64, CT_F64, A + s * Dr, CT_F64, R, CT_F64, (double*)y + s * Dl * Dr, q));
     }
     void* g;
     CT_CHECK(ws_get(h, WS_TMP3, (size_t)nG * 8, &g));
-    hipLaunchKernelGGL(adj_gram_kernel, dim3((unsigned)nG), dim3(256), 0, h->stream, (const double*)t1,
-                       (const double*)y, Dl, Dr, (int)d, (int)wR, (double*)g);
-    CT_HIP(hipGetLastError());
+    CT_CHECK(adj_gram(h, (const double*)t1, (const double*)y, Dl, Dr, d, wL, wR, (double*)g));
     // G[wl][s][wr][s'] -> gW[wl][s][s'][wr]
     const int64_t shape[4] = {wL, d, wR, d};
     const int perm[4] = {0, 1, 3, 2};
