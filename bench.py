@@ -5,7 +5,8 @@ dimension D = 1024 - the precision configuration of the reference's
 benchmark set): one "step" is one two-site H_eff matvec
 (apply_two_site_heff, reference src/algorithms/dmrg.mojo:254-374) on
 HBM-resident L, R, W and theta.  The matvec is timed in FP64 (the
-oracle-equivalent path), FP32 and 3xBF16; ``value`` is the FP32 matvec rate
+oracle-equivalent path), FP32, 3xBF16 and 6xBF16 (FP32-accurate products on
+the BF16 matrix cores); ``value`` is the FP32 matvec rate
 (the reference computes in float32) in algorithmic TFLOP/s (reference
 contraction order, 139.59 GFLOP per matvec) of the prepared operator the
 Lanczos uses (W_i W_{i+1} and the L/R copies built once per local problem);
@@ -14,13 +15,13 @@ the one-shot rate, which rebuilds them every call, is reported beside it.
 Also reported: the roofline of the dominant kernel of each form (the last
 GEMM of the matvec, U[(Dl d d)][(wR Dr)] R[(wR Dr)][Dr']) timed live with HIP
 events on the stream it runs on; fixed-D D=1024 DMRG sweeps in FP64 and in
-the refined FP32 / 3xBF16 forms on two chains - one whose local problems are
+the refined FP32 / 3xBF16 / 6xBF16 forms on two chains - one whose local problems are
 converged after the first sweep (Majumdar-Ghosh point) and one whose Lanczos
 runs its whole Krylov space (antiferromagnetic Heisenberg); the symmetric
 eigensolver of the truncation; the matvec grid over D and w; the environment
 updates and the theta build at D=1024; whole DMRG runs
 of two benchmark configurations with per-stage times (XXZ antiferromagnet L=64,
-chi <= 256, FP64; J1-J2 L=100, chi <= 1024, all three forms); and the FP64 CPU
+chi <= 256, FP64; J1-J2 L=100, chi <= 1024, all four forms); and the FP64 CPU
 oracle timed on the host cores.
 
 Prints one JSON line.  N > 1 GPUs is not supported (the path runs on one GPU).
@@ -103,7 +104,8 @@ def bench_matvec(ctx, D, w, steps, warmup):
     F = matvec_flops(D, 2, D, w, w, w)
     res = {}
     outs = {}
-    for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3)):
+    for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3),
+                       ("bf16x6", ct.PREC_BF16X6)):
         out_holder = {}
 
         # one-shot apply_two_site_heff: W_i W_{i+1} and the L/R copies are rebuilt every call
@@ -121,7 +123,7 @@ def bench_matvec(ctx, D, w, steps, warmup):
         log(f"matvec {name}: {t * 1e3:.3f} ms  {F / t / 1e12:.2f} TFLOP/s (one-shot incl. environment "
             f"preparation: {t1 * 1e3:.3f} ms, {F / t1 / 1e12:.2f} TFLOP/s)")
     ref = outs["fp64"]
-    for name in ("fp32", "bf16x3"):
+    for name in ("fp32", "bf16x3", "bf16x6"):
         res[name]["rel_err_vs_fp64"] = float(np.linalg.norm(outs[name] - ref) / np.linalg.norm(ref))
     del tL, tR, tth
     torch.cuda.empty_cache()
@@ -262,8 +264,20 @@ def bench_dominant_gemm(ctx, D, w, steps, warmup):
                                          N), steps, warmup, WARM_S)
     out["bf16x3"] = {"s": t, "tflops": F / t / 1e12, "mfma_tflops": 3 * F / t / 1e12,
                      "bytes": 2.0 * ld * (M + N) + 8.0 * M * N}
+    # 6xBF16: the same split rows plus the l planes, time only the NT MFMA kernels
+    ldl = (K + 31) // 32 * 32
+    Al = torch.empty(M * ldl, dtype=torch.int16, device="cuda")
+    Bl = torch.empty(N * ldl, dtype=torch.int16, device="cuda")
+    _lib.call("ct_split_bf16x6", ctx.handle, 0, vp(A64), M, K, K, 1, vp(As), ld, vp(Al), ldl)
+    _lib.call("ct_split_bf16x6", ctx.handle, 0, vp(B64), N, K, 1, N, vp(Bs), ld, vp(Bl), ldl)
+    C6 = torch.empty(M * N, dtype=torch.float64, device="cuda")
+    t, _ = time_events(lambda: _lib.call("ct_gemm_bf16x6_nt", ctx.handle, vp(As), ld, vp(Al), ldl, vp(Bs), ld, vp(Bl),
+                                         ldl, 0, vp(C6), M, N, K, N), steps, warmup, WARM_S)
+    out["bf16x6"] = {"s": t, "tflops": F / t / 1e12, "mfma_tflops": 6 * F / t / 1e12,
+                     "bytes": 3.0 * ldl * (M + N) + 8.0 * M * N}
     ref = (A64.view(M, K) @ B64.view(K, N)).reshape(-1)
     out["bf16x3"]["rel_err"] = float(torch.linalg.norm(C32 - ref) / torch.linalg.norm(ref))
+    out["bf16x6"]["rel_err"] = float(torch.linalg.norm(C6 - ref) / torch.linalg.norm(ref))
     out["fp64"]["rel_err"] = float(torch.linalg.norm(tC.storage - ref) / torch.linalg.norm(ref))
     out["fp32"]["rel_err"] = float(torch.linalg.norm(tC32.storage.double() - ref) / torch.linalg.norm(ref))
     for k, v in out.items():
@@ -275,7 +289,7 @@ def bench_dominant_gemm(ctx, D, w, steps, warmup):
         _lib.call("ct_scale", ctx.handle, 0, 1.0, vp(cal), cal.numel())
     torch.cuda.synchronize()
     del cal
-    del A64, B64, As, Bs, C32, tC, tA32, tB32, tC32
+    del A64, B64, As, Bs, Al, Bl, C32, C6, tC, tA32, tB32, tC32
     torch.cuda.empty_cache()
     return out, F
 
@@ -371,8 +385,8 @@ def config2_sweeps(ctx, modes):
 def config3_sweeps(ctx):
     """J1-J2 chain (J1=1, J2=0.5, w=8) L=100, PCG32 random start D=32,
     chi_max=1024, eps_trunc=1e-12, 8 sweeps, 20 Krylov vectors, converged
-    tridiagonal solves; FP64 vs FP32 and 3xBF16 matvecs with FP64 refinement.
-    J2=0.5 is the Majumdar-Ghosh point, E0 = -3L/2 = -150 exactly."""
+    tridiagonal solves; FP64 vs FP32, 3xBF16 and 6xBF16 matvecs with FP64
+    refinement.  J2=0.5 is the Majumdar-Ghosh point, E0 = -3L/2 = -150 exactly."""
     import chemtensor_mojo_amd as ct
     from chemtensor_mojo_amd import hamiltonians as ham
     L, chi, sweeps = 100, 1024, 8
@@ -380,7 +394,7 @@ def config3_sweeps(ctx):
     mps_t = ct.random_mps_c_compatible_tensors(L, 2, 32, 42)
     res = {}
     for name, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
-                               ("bf16x3_refined", ct.PREC_BF16X3, 2)):
+                               ("bf16x3_refined", ct.PREC_BF16X3, 2), ("bf16x6_refined", ct.PREC_BF16X6, 2)):
         p = ct.DMRGParams(num_sweeps=1, chi_max=chi, eps_trunc=1e-12, max_krylov_iter=20, krylov_tol=1e-12,
                           energy_tol=0.0, prec=prec, refine=refine, jacobi_max_rot=-1)
         run = ct.DMRGRun(ctx, ct.mpo_from_tensors(ctx, mpo_t), ct.mps_from_tensors(ctx, mps_t), p)
@@ -697,14 +711,16 @@ def study_eigensolver(ctx, args):
 
 
 def bench_sweeps(ctx, args):
-    """Fixed-D sweeps in FP64 and the refined FP32 / 3xBF16 forms on the two chains (see the module docstring)."""
+    """Fixed-D sweeps in FP64 and the refined FP32 / 3xBF16 / 6xBF16 forms on the two chains (see the module
+    docstring)."""
     import chemtensor_mojo_amd as ct
     sweep = {}
     for key, model, L_sites, nsw in (("j1j2_mg", "j1j2", args.sweep_sites, 2),
                                      ("heisenberg_afm", "heisenberg", args.afm_sites, 2)):
         runs = {name: bench_sweep(ctx, L_sites, args.sweep_D, prec, refine, model=model, sweeps=nsw)
                 for name, prec, refine in (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2),
-                                           ("bf16x3_refined", ct.PREC_BF16X3, 2))}
+                                           ("bf16x3_refined", ct.PREC_BF16X3, 2),
+                                           ("bf16x6_refined", ct.PREC_BF16X6, 2))}
         E64 = runs["fp64"]["energies"][-1]
         desc = ("J1-J2 chain at the Majumdar-Ghosh point (J2=0.5, w=8): after the first sweep every local "
                 "problem is converged, so the sweep times the truncation and a one-vector Lanczos"
@@ -721,6 +737,9 @@ def bench_sweeps(ctx, args):
             "speedup_sweep_bf16x3": [a / b for a, b in zip(runs["fp64"]["sweep_s"],
                                                            runs["bf16x3_refined"]["sweep_s"])],
             "speedup_sweep_fp32": [a / b for a, b in zip(runs["fp64"]["sweep_s"], runs["fp32_refined"]["sweep_s"])],
+            "dE_rel_bf16x6_vs_fp64": abs(runs["bf16x6_refined"]["energies"][-1] - E64) / abs(E64),
+            "speedup_sweep_bf16x6": [a / b for a, b in zip(runs["fp64"]["sweep_s"],
+                                                           runs["bf16x6_refined"]["sweep_s"])],
         }
     return sweep
 
@@ -862,8 +881,8 @@ def main():
                    "matvec_gflop": round(F / 1e9, 3),
                    "value_basis": "FP32 matvec (the reference's float32 arithmetic) of the prepared H_eff operator "
                                   "(W_i W_i+1 and the L/R copies built once per local problem, as inside the "
-                                  "Lanczos); FP64 and 3xBF16 forms and the one-shot figures in matvec"},
-        "matvec": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                                  "Lanczos); FP64, 3xBF16 and 6xBF16 forms and the one-shot figures in matvec"},
+        "matvec": {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
                    for k, v in mv.items()},
         "roofline": {"bound": "mfma", "kernel": dom_kernel,
                      "achieved": round(dom["tflops"], 3), "peak": PEAK["fp32"], "unit": "TFLOP/s",
@@ -882,6 +901,16 @@ def main():
                             "note": "achieved counts the algorithmic 2*M*N*K once although three BF16 products "
                                     "run per multiply-add; mfma_issue_frac counts all three against the dense "
                                     "BF16 peak"},
+        "roofline_bf16x6": {"bound": "mfma", "kernel": "gemm_bf16x6_dma_kernel + splitk_reduce4_kernel (same GEMM, "
+                                                       "256x128 tiles, split-K 2)",
+                            "achieved": round(gm["bf16x6"]["tflops"], 3), "peak": round(PEAK["bf16"] / 6.0, 1),
+                            "unit": "TFLOP/s", "frac": round(6.0 * gm["bf16x6"]["tflops"] / PEAK["bf16"], 4),
+                            "rel_err": float(f"{gm['bf16x6']['rel_err']:.3g}"),
+                            "rel_err_fp32": float(f"{gm['fp32']['rel_err']:.3g}"),
+                            "vs_fp32_peak": round(gm["bf16x6"]["tflops"] / PEAK["fp32"], 4),
+                            "note": "FP32-accurate GEMM on the BF16 matrix cores: six BF16 products per "
+                                    "multiply-add; peak = dense BF16 peak / 6, so frac is the MFMA issue "
+                                    "fraction; rel_err against the FP64 product, beside the FP32 MFMA GEMM's"},
         "sweep": sweep,
         "configs": configs,
         "eigensolver": {"workload": bench_eigensolver.__doc__.replace("\n", " ").replace("    ", " "), "by_n": eig},

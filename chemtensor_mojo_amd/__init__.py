@@ -1,11 +1,11 @@
 """chemtensor_mojo_amd: the two-site DMRG hot path of chemtensor_mojo on MI355X (gfx950).
 
 Everything numeric runs in libdmrg_hip.so (hand-written HIP kernels for CDNA4:
-MFMA GEMMs in FP64 / FP32 / 3xBF16, MPO mixing, device Lanczos, rocSOLVER
+MFMA GEMMs in FP64 / FP32 / 3xBF16 / 6xBF16, MPO mixing, device Lanczos, rocSOLVER
 SVD, C++ sweep driver).  The Python modules mirror the reference's Mojo call
 surface (src/m_tensor, src/state, src/algorithms) on top of the C ABI.
 """
-from ._lib import PREC_BF16X3, PREC_FP32, PREC_FP64, SVD_GESVD, SVD_GRAM
+from ._lib import PREC_BF16X3, PREC_BF16X6, PREC_FP32, PREC_FP64, SVD_GESVD, SVD_GRAM
 from .dense_tensor import (DenseTensor, DeviceContext, axpy_in_place, compute_row_major_strides, create_dense_tensor,
                            create_dense_tensor_from_data, create_dense_tensor_uninitialized, dense_tensor_dot,
                            dot_product)

@@ -15,7 +15,7 @@ import weakref
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdmrg_hip.so")
 
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_BF16X6 = 0, 1, 2, 3
 F64, F32 = 0, 1
 SVD_GESVD, SVD_GRAM = 0, 1
 
@@ -56,6 +56,8 @@ SIGNATURES = {
     "ct_heff_destroy": [_vp],
     "ct_split_bf16": [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64],
     "ct_gemm_bf16x3_nt": [_vp, _vp, _i64, _vp, _i64, _int, _vp, _i64, _i64, _i64, _i64],
+    "ct_split_bf16x6": [_vp, _int, _vp, _i64, _i64, _i64, _i64, _vp, _i64, _vp, _i64],
+    "ct_gemm_bf16x6_nt": [_vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, _int, _vp, _i64, _i64, _i64, _i64],
     "ct_permute": [_vp, _int, _vp, _int, _i64p, _ip, _vp],
     "ct_norm_sq": [_vp, _int, _vp, _i64, _dp],
     "ct_dot": [_vp, _int, _vp, _vp, _i64, _dp],

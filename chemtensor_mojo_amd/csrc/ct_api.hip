@@ -157,7 +157,7 @@ int ct_tensordot(ct_handle_t h, int prec, int dtype, const void* A, int rankA, c
                  int rankB, const int64_t* shapeB, int ndim_mult, int axrange_A, int axrange_B, void* C, int rankC,
                  const int64_t* shapeC) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
-    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(prec >= 0 && prec <= CT_PREC_BF16X6, 2, "unknown precision");
     CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
     return ct::tensordot(h, prec, dtype, A, rankA, shapeA, B, rankB, shapeB, ndim_mult, axrange_A, axrange_B, C,
                          rankC, shapeC);
@@ -187,6 +187,39 @@ int ct_gemm_bf16x3_nt(ct_handle_t h, const uint16_t* A, int64_t lda, const uint1
     a.p = A; a.ld = lda;
     b.p = B; b.ld = ldb;
     return ct::gemm_bf16x3_nt(h, a, b, dtype_c, C, (int)M, (int)N, (int)K, ldc, 0, 1, 1.0, 0.0, nullptr);
+}
+
+int ct_split_bf16x6(ct_handle_t h, int dtype, const void* X, int64_t rows, int64_t K, int64_t s_row, int64_t s_k,
+                    uint16_t* out, int64_t ld, uint16_t* outl, int64_t ldl) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(dtype == CT_F64 || dtype == CT_F32, 2, "unknown dtype");
+    CT_ARG(rows >= 0 && rows < (1ll << 31), 4, "rows out of range");
+    CT_ARG(K >= 0 && K < (1ll << 31), 5, "K out of range");
+    CT_ARG(ld >= 2 * ct::split_k32(K), 9, "ld must be >= 2*roundup(K, 32)");
+    CT_ARG(outl != nullptr, 10, "outl is NULL");
+    CT_ARG(ldl >= ct::split_k32(K), 11, "ldl must be >= roundup(K, 32)");
+    return ct::split_bf16(h, dtype, X, (int)rows, (int)K, s_row, s_k, 1, 0, out, ld, 0, nullptr, outl, ldl, 0);
+}
+
+int ct_gemm_bf16x6_nt(ct_handle_t h, const uint16_t* A, int64_t lda, const uint16_t* Al, int64_t ldal,
+                      const uint16_t* B, int64_t ldb, const uint16_t* Bl, int64_t ldbl, int dtype_c, void* C,
+                      int64_t M, int64_t N, int64_t K, int64_t ldc) {
+    CT_ARG(h != nullptr, 1, "handle is NULL");
+    CT_ARG(lda >= 2 * ct::split_k32(K), 3, "lda must be >= 2*roundup(K, 32)");
+    CT_ARG(Al != nullptr, 4, "Al is NULL");
+    CT_ARG(ldal >= ct::split_k32(K), 5, "ldal must be >= roundup(K, 32)");
+    CT_ARG(ldb >= 2 * ct::split_k32(K), 7, "ldb must be >= 2*roundup(K, 32)");
+    CT_ARG(Bl != nullptr, 8, "Bl is NULL");
+    CT_ARG(ldbl >= ct::split_k32(K), 9, "ldbl must be >= roundup(K, 32)");
+    CT_ARG(dtype_c == CT_F64 || dtype_c == CT_F32, 10, "unknown dtype");
+    CT_ARG(M >= 0 && M < (1ll << 31), 12, "M out of range");
+    CT_ARG(N >= 0 && N < (1ll << 31), 13, "N out of range");
+    CT_ARG(K >= 0 && K < (1ll << 31), 14, "K out of range");
+    CT_ARG(ldc >= N, 15, "ldc must be >= N");
+    ct::SplitOp a, b;
+    a.p = A; a.ld = lda; a.l = Al; a.ldl = ldal;
+    b.p = B; b.ld = ldb; b.l = Bl; b.ldl = ldbl;
+    return ct::gemm_bf16x6_nt(h, a, b, dtype_c, C, (int)M, (int)N, (int)K, ldc, 0, 1, 1.0, 0.0, nullptr);
 }
 
 int ct_permute(ct_handle_t h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm, void* out) {
@@ -232,7 +265,7 @@ int ct_heff_apply(ct_handle_t h, int prec, int dtype, const void* L, const void*
                   const void* theta, void* out, int64_t Dl, int64_t Dlb, int64_t d, int64_t Dr, int64_t Drb,
                   int64_t wL, int64_t wM, int64_t wR) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
-    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(prec >= 0 && prec <= CT_PREC_BF16X6, 2, "unknown precision");
     CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
     CT_ARG(Dl > 0 && Dlb > 0 && d > 0 && Dr > 0 && Drb > 0 && wL > 0 && wM > 0 && wR > 0, 11, "dimensions must be > 0");
     ct::HeffDims hd{Dl, Dlb, d, Dr, Drb, wL, wM, wR};
@@ -255,7 +288,7 @@ int ct_heff_create(ct_handle_t h, int prec, int dtype, const void* L, const void
                    int64_t Dl, int64_t Dlb, int64_t d, int64_t Dr, int64_t Drb, int64_t wL, int64_t wM, int64_t wR,
                    ct_heff_t* op) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
-    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(prec >= 0 && prec <= CT_PREC_BF16X6, 2, "unknown precision");
     CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
     CT_ARG(Dl > 0 && Dlb > 0 && d > 0 && Dr > 0 && Drb > 0 && wL > 0 && wM > 0 && wR > 0, 8, "dimensions must be > 0");
     CT_ARG(op != nullptr, 16, "op is NULL");
@@ -296,7 +329,7 @@ int ct_heff_destroy(ct_heff_t op) {
 int ct_env_update_left(ct_handle_t h, int prec, int dtype, const void* L, const void* A, const void* W, void* out,
                        int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
-    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(prec >= 0 && prec <= CT_PREC_BF16X6, 2, "unknown precision");
     CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
     CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wR > 0, 8, "dimensions must be > 0");
     return ct::env_left(h, prec, dtype, L, A, W, out, Dl, d, Dr, wL, wR);
@@ -305,7 +338,7 @@ int ct_env_update_left(ct_handle_t h, int prec, int dtype, const void* L, const 
 int ct_env_update_right(ct_handle_t h, int prec, int dtype, const void* R, const void* A, const void* W, void* out,
                         int64_t Dl, int64_t d, int64_t Dr, int64_t wL, int64_t wR) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
-    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(prec >= 0 && prec <= CT_PREC_BF16X6, 2, "unknown precision");
     CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
     CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wR > 0, 8, "dimensions must be > 0");
     return ct::env_right(h, prec, dtype, R, A, W, out, Dl, d, Dr, wL, wR);
@@ -328,7 +361,7 @@ int ct_lanczos_two_site(ct_handle_t h, int prec, int dtype, const void* L, const
                         int64_t wR, int max_iter, double tol, int max_rotations, int refine, double* E0,
                         int* num_iter, double* E_lanczos, double* residual_norm) {
     CT_ARG(h != nullptr, 1, "handle is NULL");
-    CT_ARG(prec >= 0 && prec <= 2, 2, "unknown precision");
+    CT_ARG(prec >= 0 && prec <= CT_PREC_BF16X6, 2, "unknown precision");
     CT_ARG(dtype == CT_F64 || dtype == CT_F32, 3, "unknown dtype");
     CT_ARG(Dl > 0 && d > 0 && Dr > 0 && wL > 0 && wM > 0 && wR > 0, 10, "dimensions must be > 0");
     CT_ARG(max_iter >= 1 && max_iter <= 64, 16, "max_iter must be in [1, 64]");

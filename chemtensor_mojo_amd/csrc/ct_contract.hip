@@ -134,22 +134,32 @@ int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, co
         env->R = r32;
         return 0;
     }
-    if (prec == CT_PREC_BF16X3) {
+    if (prec == CT_PREC_BF16X3 || prec == CT_PREC_BF16X6) {
+        const bool x6 = prec == CT_PREC_BF16X6;
         // Ls: split rows [wL][Dlb][k = Dl] from L[wL][Dl][Dlb]: row stride 1, k stride Dlb
-        const int64_t ldL = 2 * split_k32(hd.Dl), bsL = hd.Dlb * ldL, nL = hd.wL * bsL;
+        const int64_t KL32 = split_k32(hd.Dl), ldL = 2 * KL32, bsL = hd.Dlb * ldL, nL = hd.wL * bsL;
         // Rs: split rows [Drb][k = wR*Dr] from R[(wR Dr)][Drb]: row stride 1, k stride Drb
-        const int64_t KR = hd.wR * hd.Dr, ldR = 2 * split_k32(KR), nR = hd.Drb * ldR;
-        void *ls, *rs;
+        const int64_t KR = hd.wR * hd.Dr, KR32 = split_k32(KR), ldR = 2 * KR32, nR = hd.Drb * ldR;
+        void *ls, *rs, *ll = nullptr, *rl = nullptr;
         CT_CHECK(get(WS_LSPLIT, store ? &store->l : nullptr, (size_t)nL * 2, &ls));
         CT_CHECK(get(WS_RSPLIT, store ? &store->r : nullptr, (size_t)nR * 2, &rs));
+        if (x6) {  // the l planes: half the split rows' size
+            CT_CHECK(get(WS_LSPLIT_L, store ? &store->ll : nullptr, (size_t)nL, &ll));
+            CT_CHECK(get(WS_RSPLIT_L, store ? &store->rl : nullptr, (size_t)nR, &rl));
+        }
         CT_CHECK(split_bf16(h, tEnv, L, (int)hd.Dlb, (int)hd.Dl, 1, hd.Dlb, (int)hd.wL, hd.Dl * hd.Dlb,
-                            (uint16_t*)ls, ldL, bsL, nullptr));
-        CT_CHECK(split_bf16(h, tEnv, R, (int)hd.Drb, (int)KR, 1, hd.Drb, 1, 0, (uint16_t*)rs, ldR, 0, nullptr));
+                            (uint16_t*)ls, ldL, bsL, nullptr, (uint16_t*)ll, KL32, hd.Dlb * KL32));
+        CT_CHECK(split_bf16(h, tEnv, R, (int)hd.Drb, (int)KR, 1, hd.Drb, 1, 0, (uint16_t*)rs, ldR, 0, nullptr,
+                            (uint16_t*)rl, KR32, 0));
         env->tEnv = tEnv;
         env->L = L;
         env->R = R;
         env->Ls.p = (const uint16_t*)ls; env->Ls.ld = ldL; env->Ls.bs = bsL;
         env->Rs.p = (const uint16_t*)rs; env->Rs.ld = ldR; env->Rs.bs = 0;
+        if (x6) {
+            env->Ls.l = (const uint16_t*)ll; env->Ls.ldl = KL32; env->Ls.bsl = hd.Dlb * KL32;
+            env->Rs.l = (const uint16_t*)rl; env->Rs.ldl = KR32; env->Rs.bsl = 0;
+        }
         return 0;
     }
     set_last_error("unknown precision");
@@ -169,31 +179,49 @@ int heff_apply(Handle* h, const HeffEnv& env, int tVec, const void* theta, void*
     m.sYc = D.Dr; m.sYp = d2 * D.wR * D.Dr; m.sYe = D.wR * D.Dr;
     m.active = active;
 
-    if (env.prec == CT_PREC_BF16X3) {
-        // theta [Dl][(d d Dr)] -> split rows [n = d d Dr][k = Dl]
-        const int64_t Nn = d2 * D.Dr, ldV = 2 * split_k32(D.Dl), nV = Nn * ldV;
-        void *vs, *t1, *us;
+    if (env.prec == CT_PREC_BF16X3 || env.prec == CT_PREC_BF16X6) {
+        const bool x6 = env.prec == CT_PREC_BF16X6;
+        auto gemm_split = x6 ? gemm_bf16x6_nt : gemm_bf16x3_nt;
+        // theta [Dl][(d d Dr)] -> split rows [n = d d Dr][k = Dl] (6xBF16: and its l plane)
+        const int64_t Nn = d2 * D.Dr, KV32 = split_k32(D.Dl), ldV = 2 * KV32, nV = Nn * ldV;
+        void *vs, *t1, *us, *vl = nullptr, *ul = nullptr;
         CT_CHECK(ws_get(h, WS_VSPLIT, (size_t)nV * 2, &vs));
         CT_CHECK(ws_get(h, WS_T1, (size_t)nT1 * 4, &t1));
-        const int64_t KU = D.wR * D.Dr, ldU = 2 * split_k32(KU), nUs = D.Dlb * d2 * ldU;
+        const int64_t KU = D.wR * D.Dr, KU32 = split_k32(KU), ldU = 2 * KU32, nUs = D.Dlb * d2 * ldU;
         CT_CHECK(ws_get(h, WS_USPLIT, (size_t)nUs * 2, &us));
-        CT_CHECK(split_bf16(h, tVec, theta, (int)Nn, (int)D.Dl, 1, Nn, 1, 0, (uint16_t*)vs, ldV, 0, active));
+        if (x6) {
+            CT_CHECK(ws_get(h, WS_VSPLIT_L, (size_t)nV, &vl));
+            CT_CHECK(ws_get(h, WS_USPLIT_L, (size_t)nUs, &ul));
+        }
+        CT_CHECK(split_bf16(h, tVec, theta, (int)Nn, (int)D.Dl, 1, Nn, 1, 0, (uint16_t*)vs, ldV, 0, active,
+                            (uint16_t*)vl, KV32, 0));
         SplitOp sv;
         sv.p = (const uint16_t*)vs; sv.ld = ldV; sv.bs = 0;
+        sv.l = (const uint16_t*)vl; sv.ldl = KV32; sv.bsl = 0;
         // T1[b][Dlb][(d d Dr)] = Ls[b] (Dlb x Dl) . Vs^T
-        CT_CHECK(gemm_bf16x3_nt(h, env.Ls, sv, CT_F32, t1, (int)D.Dlb, (int)Nn, (int)D.Dl, Nn, D.Dlb * Nn,
-                                (int)D.wL, 1.0, 0.0, active));
-        // U written as split rows [(Dlb d d)][k = (wR Dr)]
+        CT_CHECK(gemm_split(h, env.Ls, sv, CT_F32, t1, (int)D.Dlb, (int)Nn, (int)D.Dl, Nn, D.Dlb * Nn, (int)D.wL, 1.0,
+                            0.0, active));
+        // U written as split rows [(Dlb d d)][k = (wR Dr)] (6xBF16: and its l plane)
         uint16_t* uh = (uint16_t*)us;
         MixDesc ms = m;
         ms.sYc = D.Dr; ms.sYp = d2 * ldU; ms.sYe = ldU;
-        CT_CHECK(split_zero_pad(h, uh, D.Dlb * d2, KU, ldU));
-        CT_CHECK(mix_split(h, (const float*)t1, uh, env.W2, ms, env.W2pad));
+        CT_CHECK(split_zero_pad(h, uh, D.Dlb * d2, KU, ldU, (uint16_t*)ul, KU32));
+        if (x6 && mix_desc_is_large(ms)) {
+            // large MPO bonds: FP32 mixing (the batched-GEMM path), then the 6xBF16 split of U
+            void* u32;
+            CT_CHECK(ws_get(h, WS_T2, (size_t)nU * 4, &u32));
+            CT_CHECK(mix(h, CT_F32, t1, CT_F32, u32, env.W2, m));
+            CT_CHECK(split_bf16(h, CT_F32, u32, (int)(D.Dlb * d2), (int)KU, KU, 1, 1, 0, uh, ldU, 0, active,
+                                (uint16_t*)ul, KU32, 0));
+        } else {
+            CT_CHECK(mix_split(h, (const float*)t1, uh, env.W2, ms, env.W2pad, (uint16_t*)ul));
+        }
         SplitOp su;
         su.p = uh; su.ld = ldU; su.bs = 0;
+        su.l = (const uint16_t*)ul; su.ldl = KU32; su.bsl = 0;
         // out[(Dlb d d)][Drb] = Us . Rs^T
-        return gemm_bf16x3_nt(h, su, env.Rs, tVec, out, (int)(D.Dlb * d2), (int)D.Drb, (int)KU, D.Drb, 0, 1, 1.0,
-                              0.0, active);
+        return gemm_split(h, su, env.Rs, tVec, out, (int)(D.Dlb * d2), (int)D.Drb, (int)KU, D.Drb, 0, 1, 1.0, 0.0,
+                          active);
     }
 
     const int tMid = env.prec == CT_PREC_FP64 ? CT_F64 : CT_F32;

@@ -258,8 +258,8 @@ int ct_dmrg_create(ct_handle_t h, const ct_dmrg_params* params, int nsites, int6
     CT_ARG(dtype == CT_F64 || dtype == CT_F32, 5, "unknown dtype");
     CT_ARG(params->chi_max >= 1, 2, "chi_max must be >= 1");
     CT_ARG(params->max_krylov_iter >= 1 && params->max_krylov_iter <= 64, 2, "max_krylov_iter must be in [1, 64]");
-    CT_ARG(params->prec >= 0 && params->prec <= 2, 2, "unknown precision");
-    CT_ARG(params->env_prec >= 0 && params->env_prec <= 2, 2, "unknown environment precision");
+    CT_ARG(params->prec >= 0 && params->prec <= CT_PREC_BF16X6, 2, "unknown precision");
+    CT_ARG(params->env_prec >= 0 && params->env_prec <= CT_PREC_BF16X6, 2, "unknown environment precision");
     CT_ARG(params->jacobi_max_rot >= -1, 2, "jacobi_max_rot must be >= -1");
     CT_ARG(params->svd_mode == CT_SVD_GESVD || params->svd_mode == CT_SVD_GRAM, 2, "unknown svd_mode");
     ct_dmrg_s* s = new ct_dmrg_s();

@@ -439,8 +439,6 @@ int dispatch_io(Handle* h, int tA, const void* A, int tB, const void* B, int tC,
 
 }  // namespace
 
-int gemm_bf16x3(Handle* h, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g);
-
 int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g) {
     if (g.M <= 0 || g.N <= 0 || g.batch <= 0) return 0;
     if (g.K <= 0) {
@@ -465,6 +463,7 @@ int gemm(Handle* h, int prec, int tA, const void* A, int tB, const void* B, int 
     }
     if (prec == CT_PREC_FP32) return dispatch_io<float>(h, tA, A, tB, B, tC, C, g);
     if (prec == CT_PREC_BF16X3) return gemm_bf16x3(h, tA, A, tB, B, tC, C, g);
+    if (prec == CT_PREC_BF16X6) return gemm_bf16x6(h, tA, A, tB, B, tC, C, g);
     set_last_error("unknown precision");
     return CT_STATUS_NOT_SUPPORTED;
 }

@@ -56,6 +56,17 @@ __device__ __forceinline__ void split_one(T v, uint16_t& hi, uint16_t& lo) {
     lo = f32_to_bf16_rne(x - bf16_to_f32(hi));
 }
 
+// Three-way split of the FP32 value: hi + lo + l carries all of its 24
+// significand bits (x - hi and x - hi - lo are exact in FP32).
+template <typename T>
+__device__ __forceinline__ void split_three(T v, uint16_t& hi, uint16_t& lo, uint16_t& l) {
+    const float x = (float)v;
+    hi = f32_to_bf16_rne(x);
+    const float r = x - bf16_to_f32(hi);
+    lo = f32_to_bf16_rne(r);
+    l = f32_to_bf16_rne(r - bf16_to_f32(lo));
+}
+
 // ---------------------------------------------------------------- split
 // Strided split into split rows.  One thread per (row, k) of a 32 (k) x 32
 // (rows) block, 256 threads as 32 x 8; the block stages through LDS so both
@@ -64,7 +75,8 @@ __device__ __forceinline__ void split_one(T v, uint16_t& hi, uint16_t& lo) {
 template <typename T>
 __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ X, int rows, int K, int64_t s_row,
                                                     int64_t s_k, int64_t bsX, uint16_t* __restrict__ out, int64_t ld,
-                                                    int64_t bs, const int* active) {
+                                                    int64_t bs, const int* active, uint16_t* __restrict__ outl,
+                                                    int64_t ldl, int64_t bsl) {
     if (active && *active == 0) return;
     __shared__ float tile[32][33];
     const int b = blockIdx.z;
@@ -72,6 +84,7 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ X, int
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     X += (int64_t)b * bsX;
     out += (int64_t)b * bs;
+    if (outl) outl += (int64_t)b * bsl;
     if (s_row == 1) {
         // rows contiguous: read with tx along rows, write with tx along k
 #pragma unroll
@@ -84,11 +97,12 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ X, int
         for (int i = 0; i < 4; ++i) {
             const int r = r0 + ty + 8 * i;
             if (r < rows) {
-                uint16_t h, l;
-                split_one(tile[tx][ty + 8 * i], h, l);
+                uint16_t h, l, l2;
+                split_three(tile[tx][ty + 8 * i], h, l, l2);
                 uint16_t* o = out + (int64_t)r * ld + 2 * k0 + tx;
                 o[0] = h;
                 o[32] = l;
+                if (outl) outl[(int64_t)r * ldl + k0 + tx] = l2;
             }
         }
     } else {
@@ -96,11 +110,12 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ X, int
         for (int i = 0; i < 4; ++i) {
             const int r = r0 + ty + 8 * i, k = k0 + tx;
             if (r < rows) {
-                uint16_t h = 0, l = 0;
-                if (k < K) split_one(X[(int64_t)r * s_row + (int64_t)k * s_k], h, l);
+                uint16_t h = 0, l = 0, l2 = 0;
+                if (k < K) split_three(X[(int64_t)r * s_row + (int64_t)k * s_k], h, l, l2);
                 uint16_t* o = out + (int64_t)r * ld + 2 * k0 + tx;
                 o[0] = h;
                 o[32] = l;
+                if (outl) outl[(int64_t)r * ldl + k] = l2;
             }
         }
     }
@@ -115,7 +130,8 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ X, int
 template <typename T, bool TWO_LEVEL>
 __global__ __launch_bounds__(256) void split_t_kernel(const T* __restrict__ X, int rows, int K, int kin, int64_t s_k,
                                                       int64_t s_kc, int64_t bsX, uint16_t* __restrict__ out,
-                                                      int64_t ld, int64_t bs, const int* active) {
+                                                      int64_t ld, int64_t bs, const int* active,
+                                                      uint16_t* __restrict__ outl, int64_t ldl, int64_t bsl) {
     if (active && *active == 0) return;
     __shared__ float tile[32][65];
     const int b = blockIdx.z;
@@ -123,6 +139,7 @@ __global__ __launch_bounds__(256) void split_t_kernel(const T* __restrict__ X, i
     const int t = threadIdx.x;
     X += (int64_t)b * bsX;
     out += (int64_t)b * bs;
+    if (outl) outl += (int64_t)b * bsl;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int kk = (t >> 6) + 4 * i, k = k0 + kk, r = r0 + (t & 63);
@@ -133,18 +150,20 @@ __global__ __launch_bounds__(256) void split_t_kernel(const T* __restrict__ X, i
     const int row = t >> 2, grp = t & 3;
     const int r = r0 + row;
     if (r < rows) {
-        uint32_t hp[4], lp[4];
+        uint32_t hp[4], lp[4], mp[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            uint16_t h0, l0, h1, l1;
-            split_one(tile[grp * 8 + 2 * q][row], h0, l0);
-            split_one(tile[grp * 8 + 2 * q + 1][row], h1, l1);
+            uint16_t h0, l0, m0, h1, l1, m1;
+            split_three(tile[grp * 8 + 2 * q][row], h0, l0, m0);
+            split_three(tile[grp * 8 + 2 * q + 1][row], h1, l1, m1);
             hp[q] = (uint32_t)h0 | ((uint32_t)h1 << 16);
             lp[q] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+            mp[q] = (uint32_t)m0 | ((uint32_t)m1 << 16);
         }
         uint16_t* o = out + (int64_t)r * ld + 2 * k0 + grp * 8;
         *(v4u*)o = v4u{hp[0], hp[1], hp[2], hp[3]};
         *(v4u*)(o + 32) = v4u{lp[0], lp[1], lp[2], lp[3]};
+        if (outl) *(v4u*)(outl + (int64_t)r * ldl + k0 + grp * 8) = v4u{mp[0], mp[1], mp[2], mp[3]};
     }
 }
 
@@ -358,6 +377,246 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_dma_kernel(NtArgs p) {
     }
 }
 
+// ------------------------------------------------- 6xBF16, 256x128 tiling
+// FP32-accurate products on the BF16 matrix cores.  With x = h + m + l
+// (h, m: the split rows' hi and lo; l: the l plane), the six products
+//   l*H + h*L + m*M + m*H + h*M + h*H        (smallest terms first)
+// leave out only m*L + l*M + l*L, below 2^-24 |a b| (the bf16x6 scheme of
+// Henry, Tang and Heinecke, "Leveraging the bfloat16 artificial intelligence
+// datatype for higher-precision computations", ARITH 2019), so with FP32
+// accumulation the result carries FP32 GEMM accuracy.
+// 256 (m) x 128 (n) x 32 (k) blocks, 256 threads = 4 waves (2 x 2) of 128 x 64
+// with AGPR accumulators (h*H and the five smaller products in separate
+// accumulators, summed once at the end).  Per stage the split rows (128 B per row) and the l
+// planes (64 B per row) of both tiles travel by direct-to-LDS loads: 72 KiB,
+// two stages.  Inside an l-plane row, 16-byte chunk c sits at slot
+// c ^ lswz(r), which spreads the fragment reads of 16 consecutive rows over
+// distinct banks.
+constexpr int X6M = 256, X6N = 128;
+constexpr int X6_AS = 0;                    // A split rows: 256 x 128 B
+constexpr int X6_BS = X6_AS + X6M * SROW;   // B split rows: 128 x 128 B
+constexpr int X6_AL = X6_BS + X6N * SROW;   // A l plane: 256 x 64 B
+constexpr int X6_BL = X6_AL + X6M * 64;     // B l plane: 128 x 64 B
+constexpr int X6_STAGE = X6_BL + X6N * 64;  // 72 KiB
+
+struct X6Args {
+    NtArgs nt;  // split rows, output and tiling (tilesM, tilesN count 256 x 128 tiles)
+    const uint16_t *Al, *Bl;
+    int64_t ldal, ldbl, bsAl, bsBl;
+};
+
+__device__ __forceinline__ int lswz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+template <typename TO, bool SPLIT>
+__global__ __launch_bounds__(256) void gemm_bf16x6_dma_kernel(X6Args q) {
+    const NtArgs& p = q.nt;
+    if (p.active && *p.active == 0) return;
+    __shared__ __attribute__((aligned(1024))) char smem[2 * X6_STAGE];
+
+    int tm, tn, b, sk, z;
+    if (p.batch_xcd == 2) {  // split K: one K range and a compact block of tiles per XCD (see the 3xBF16 kernel)
+        const int nt = gridDim.x, total = nt * gridDim.z;
+        const int f = blockIdx.z * nt + blockIdx.x;
+        const int v = (f & 7) * (total >> 3) + (f >> 3);
+        z = v / nt;
+        gemm_group_coords(v - z * nt, p.tilesM, p.tilesN, p.GM, tm, tn);
+        b = z / p.splits;
+        sk = z - b * p.splits;
+    } else if (p.batch_xcd) {  // splits == 1, whole batches per XCD
+        int tile;
+        gemm_batch_per_xcd(gridDim.x, b, tile);
+        gemm_group_coords(tile, p.tilesM, p.tilesN, p.GM, tm, tn);
+        sk = 0;
+        z = b;
+    } else {
+        gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, p.GM, tm, tn);
+        z = blockIdx.z;
+        b = z / p.splits;
+        sk = z - b * p.splits;
+    }
+    const int m0 = tm * X6M, n0 = tn * X6N;
+    const int kbeg = sk * p.kchunk;
+    const int kend = min(p.K32, kbeg + p.kchunk);
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    const uint16_t* PA = p.A + (int64_t)b * p.bsA;
+    const uint16_t* PB = p.B + (int64_t)b * p.bsB;
+    const uint16_t* LA = q.Al + (int64_t)b * q.bsAl;
+    const uint16_t* LB = q.Bl + (int64_t)b * q.bsBl;
+    // Copies of wave w: split-row rows [64w, 64w+64) of A and [32w, 32w+32) of
+    // B (one instruction = 8 rows x 128 B), and the same rows of the l planes
+    // (one instruction = 16 rows x 64 B): 18 instructions per wave and stage.
+    // Rows past the matrix edge re-read the last row (never stored).
+    const int r8 = lane >> 3, s8 = lane & 7, r4 = lane >> 2, s4 = lane & 3;
+    auto issue = [&](int stage, int k0) {
+        char* base = smem + stage * X6_STAGE;
+#pragma unroll
+        for (int blk = 0; blk < 8; ++blk) {
+            const int row = 64 * wave + 8 * blk + r8;
+            const int grow = min(m0 + row, p.M - 1);
+            const uint16_t* src = PA + (int64_t)grow * p.lda + 2 * k0 + (s8 ^ (row & 7)) * 8;
+            __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)(base + X6_AS + (64 * wave + 8 * blk) * SROW),
+                                             16, 0, 0);
+        }
+#pragma unroll
+        for (int blk = 0; blk < 4; ++blk) {
+            const int row = 32 * wave + 8 * blk + r8;
+            const int grow = min(n0 + row, p.N - 1);
+            const uint16_t* src = PB + (int64_t)grow * p.ldb + 2 * k0 + (s8 ^ (row & 7)) * 8;
+            __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)(base + X6_BS + (32 * wave + 8 * blk) * SROW),
+                                             16, 0, 0);
+        }
+#pragma unroll
+        for (int blk = 0; blk < 4; ++blk) {
+            const int row = 64 * wave + 16 * blk + r4;
+            const int grow = min(m0 + row, p.M - 1);
+            const uint16_t* src = LA + (int64_t)grow * q.ldal + k0 + (s4 ^ lswz(row)) * 8;
+            __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)(base + X6_AL + (64 * wave + 16 * blk) * 64),
+                                             16, 0, 0);
+        }
+#pragma unroll
+        for (int blk = 0; blk < 2; ++blk) {
+            const int row = 32 * wave + 16 * blk + r4;
+            const int grow = min(n0 + row, p.N - 1);
+            const uint16_t* src = LB + (int64_t)grow * q.ldbl + k0 + (s4 ^ lswz(row)) * 8;
+            __builtin_amdgcn_global_load_lds((const void*)src, (lds_void*)(base + X6_BL + (32 * wave + 16 * blk) * 64),
+                                             16, 0, 0);
+        }
+    };
+
+    // acc: the h*H products; acs: the five smaller ones, so that the large
+    // running sum is rounded once per K step instead of six times
+    v4f acc[8][4], acs[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+            acs[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+
+    // One K step: the B fragments and row block 0 of A first, the A fragments
+    // of row blocks 1-7 between the MFMAs of row block 0, and only then (with
+    // every LDS read of the step issued) the direct-to-LDS copy of the next
+    // stage, between the MFMAs of row block 1.
+    if (kbeg < kend) issue(0, kbeg);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    int stage = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        const char* st = smem + stage * X6_STAGE;
+        auto rdA = [&](int i, v8bf& h, v8bf& m, v8bf& l) {
+            const int r = wm * 128 + i * 16 + li;
+            h = __builtin_bit_cast(v8bf, *(const v4u*)(st + X6_AS + r * SROW + ((lk ^ (r & 7)) * 16)));
+            m = __builtin_bit_cast(v8bf, *(const v4u*)(st + X6_AS + r * SROW + (((lk + 4) ^ (r & 7)) * 16)));
+            l = __builtin_bit_cast(v8bf, *(const v4u*)(st + X6_AL + r * 64 + ((lk ^ lswz(r)) * 16)));
+        };
+        auto rdB = [&](int j, v8bf& h, v8bf& m, v8bf& l) {
+            const int r = wn * 64 + j * 16 + li;
+            h = __builtin_bit_cast(v8bf, *(const v4u*)(st + X6_BS + r * SROW + ((lk ^ (r & 7)) * 16)));
+            m = __builtin_bit_cast(v8bf, *(const v4u*)(st + X6_BS + r * SROW + (((lk + 4) ^ (r & 7)) * 16)));
+            l = __builtin_bit_cast(v8bf, *(const v4u*)(st + X6_BL + r * 64 + ((lk ^ lswz(r)) * 16)));
+        };
+        // the last step copies its own block again (in bounds, never read)
+        const int knext = min(k0 + BK, kend - BK);
+        v8bf aH[8], aM[8], aL[8], bH[4], bM[4], bL[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rdB(j, bH[j], bM[j], bL[j]);
+        rdA(0, aH[0], aM[0], aL[0]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acs[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aL[0], bH[j], acs[0][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acs[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH[0], bL[j], acs[0][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 1; i < 8; ++i) rdA(i, aH[i], aM[i], aL[i]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acs[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aM[0], bM[j], acs[0][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acs[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aM[0], bH[j], acs[0][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acs[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH[0], bM[j], acs[0][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH[0], bH[j], acc[0][j], 0, 0, 0);
+        // 24 MFMAs of row block 0 with the 21 fragment reads of row blocks 1-7
+#pragma unroll
+        for (int qq = 0; qq < 21; ++qq) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 1; i < 8; ++i) {
+            if (i == 1) issue(stage ^ 1, knext);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aL[i], bH[j], acs[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH[i], bL[j], acs[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aM[i], bM[j], acs[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aM[i], bH[j], acs[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH[i], bM[j], acs[i][j], 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aH[i], bH[j], acc[i][j], 0, 0, 0);
+            if (i == 1) {
+                // the 18 copy instructions between the first 18 MFMAs of row block 1
+#pragma unroll
+                for (int qq = 0; qq < 18; ++qq) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one global read
+                }
+                __builtin_amdgcn_sched_group_barrier(0x008, 6, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // the next stage has landed in LDS (this wave's copies), then every wave's
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        stage ^= 1;
+    }
+
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += acs[i][j];
+    if (SPLIT) {
+        float* Pp = p.part + (int64_t)z * p.M * p.N;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + wn * 64 + j * 16 + li;
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    const int m = m0 + wm * 128 + i * 16 + lk * 4 + qq;
+                    if (m < p.M && n < p.N) Pp[(int64_t)m * p.N + n] = acc[i][j][qq];
+                }
+            }
+    } else {
+        TO* C = (TO*)p.C + (int64_t)b * p.bsC;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + wn * 64 + j * 16 + li;
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) store_out(p, C, m0 + wm * 128 + i * 16 + lk * 4 + qq, n, acc[i][j][qq]);
+            }
+    }
+}
+
 // C[b][m][n] = alpha * sum_s part[b*splits + s][m][n] + beta * C[b][m][n]
 template <typename TO>
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(const float* __restrict__ part, TO* __restrict__ C, int M,
@@ -557,26 +816,32 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
 }  // namespace
 
 int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
-               int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active) {
+               int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active, uint16_t* outl,
+               int64_t ldl, int64_t bsl) {
     if (rows <= 0 || K <= 0 || batch <= 0) return 0;
-    if (s_row == 1 && ld_out % 8 == 0 && bs_out % 8 == 0 && ((uintptr_t)out % 16) == 0) {
+    if (outl && ldl < split_k32(K)) {
+        set_last_error("split_bf16: the l plane needs ldl >= roundup(K, 32)");
+        return CT_STATUS_SHAPE;
+    }
+    const bool l_vec = !outl || (ldl % 8 == 0 && bsl % 8 == 0 && ((uintptr_t)outl % 16) == 0);
+    if (s_row == 1 && ld_out % 8 == 0 && bs_out % 8 == 0 && ((uintptr_t)out % 16) == 0 && l_vec) {
         dim3 grid((unsigned)((rows + 63) / 64), (unsigned)(split_k32(K) / 32), (unsigned)batch), block(256);
         if (tX == CT_F64)
             hipLaunchKernelGGL((split_t_kernel<double, false>), grid, block, 0, h->stream, (const double*)X, rows, K,
-                               K, s_k, (int64_t)0, bsX, out, ld_out, bs_out, active);
+                               K, s_k, (int64_t)0, bsX, out, ld_out, bs_out, active, outl, ldl, bsl);
         else
             hipLaunchKernelGGL((split_t_kernel<float, false>), grid, block, 0, h->stream, (const float*)X, rows, K, K,
-                               s_k, (int64_t)0, bsX, out, ld_out, bs_out, active);
+                               s_k, (int64_t)0, bsX, out, ld_out, bs_out, active, outl, ldl, bsl);
         CT_HIP(hipGetLastError());
         return 0;
     }
     dim3 grid((unsigned)(split_k32(K) / 32), (unsigned)((rows + 31) / 32), (unsigned)batch), block(256);
     if (tX == CT_F64)
         hipLaunchKernelGGL(split_kernel<double>, grid, block, 0, h->stream, (const double*)X, rows, K, s_row, s_k,
-                           bsX, out, ld_out, bs_out, active);
+                           bsX, out, ld_out, bs_out, active, outl, ldl, bsl);
     else
         hipLaunchKernelGGL(split_kernel<float>, grid, block, 0, h->stream, (const float*)X, rows, K, s_row, s_k,
-                           bsX, out, ld_out, bs_out, active);
+                           bsX, out, ld_out, bs_out, active, outl, ldl, bsl);
     CT_HIP(hipGetLastError());
     return 0;
 }
@@ -592,10 +857,10 @@ int split_bf16_kk(Handle* h, int tX, const void* X, int rows, int K, int kin, in
     dim3 grid((unsigned)((rows + 63) / 64), (unsigned)(split_k32(K) / 32), (unsigned)batch), block(256);
     if (tX == CT_F64)
         hipLaunchKernelGGL((split_t_kernel<double, true>), grid, block, 0, h->stream, (const double*)X, rows, K, kin,
-                           s_k, s_kc, bsX, out, ld_out, bs_out, active);
+                           s_k, s_kc, bsX, out, ld_out, bs_out, active, (uint16_t*)nullptr, (int64_t)0, (int64_t)0);
     else
         hipLaunchKernelGGL((split_t_kernel<float, true>), grid, block, 0, h->stream, (const float*)X, rows, K, kin,
-                           s_k, s_kc, bsX, out, ld_out, bs_out, active);
+                           s_k, s_kc, bsX, out, ld_out, bs_out, active, (uint16_t*)nullptr, (int64_t)0, (int64_t)0);
     CT_HIP(hipGetLastError());
     return 0;
 }
@@ -705,6 +970,70 @@ int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
     return 0;
 }
 
+int gemm_bf16x6_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* C, int M, int N, int K,
+                   int64_t ldc, int64_t bsC, int batch, double alpha, double beta, const int* active) {
+    if (M <= 0 || N <= 0 || batch <= 0) return 0;
+    const int K32 = (int)split_k32(K);
+    auto al16 = [](const void* q) { return ((uintptr_t)q % 16) == 0; };
+    if (!A.l || !B.l || A.ld < 2 * K32 || B.ld < 2 * K32 || A.ld % 8 || B.ld % 8 || A.bs % 8 || B.bs % 8 ||
+        !al16(A.p) || !al16(B.p) || A.ldl < K32 || B.ldl < K32 || A.ldl % 8 || B.ldl % 8 || A.bsl % 8 ||
+        B.bsl % 8 || !al16(A.l) || !al16(B.l)) {
+        set_last_error("6xBF16 GEMM: split rows need ld >= 2*roundup(K,32), l planes ldl >= roundup(K,32), "
+                       "strides multiples of 8 and 16-byte aligned bases");
+        return CT_STATUS_SHAPE;
+    }
+    X6Args q;
+    NtArgs& p = q.nt;
+    p.A = A.p; p.B = B.p;
+    p.C = C;
+    p.M = M; p.N = N; p.K32 = K32;
+    p.lda = A.ld; p.ldb = B.ld; p.bsA = A.bs; p.bsB = B.bs;
+    p.ldc = ldc; p.bsC = bsC;
+    p.alpha = alpha; p.beta = beta; p.active = active;
+    p.batch_xcd = 0;
+    p.part = nullptr;
+    q.Al = A.l; q.Bl = B.l;
+    q.ldal = A.ldl; q.ldbl = B.ldl; q.bsAl = A.bsl; q.bsBl = B.bsl;
+    p.tilesM = (M + X6M - 1) / X6M;
+    p.tilesN = (N + X6N - 1) / X6N;
+    // split K while the tiles leave CUs idle and every split keeps >= 16 K steps
+    const int64_t tiles = (int64_t)p.tilesM * p.tilesN * batch;
+    int splits = 1;
+    while (tiles * splits < 256 && K32 / (splits * 2) >= 512) splits *= 2;
+    const int per_xcd = (p.tilesM * p.tilesN) / 8;
+    p.GM = per_xcd >= p.tilesN ? per_xcd / p.tilesN : 1;
+    if (p.GM < 1) p.GM = 1;
+    p.splits = splits;
+    p.kchunk = splits > 1 ? (K32 / splits + BK - 1) / BK * BK : K32;
+    const int64_t total = tiles * splits;
+    if (splits > 1 && total % 8 == 0) {
+        const int64_t share = std::min<int64_t>((int64_t)p.tilesM * p.tilesN, total / 8);
+        p.batch_xcd = 2;
+        p.GM = share >= p.tilesN ? (int)(share / p.tilesN) : 1;
+    }
+    if (splits == 1 && batch % 8 == 0 && p.tilesM * p.tilesN <= 32) {
+        p.batch_xcd = 1;
+        p.GM = p.tilesM;
+    }
+    dim3 grid(p.tilesM * p.tilesN, 1, batch * splits), block(256);
+    if (splits > 1) {
+        void* w;
+        CT_CHECK(ws_get(h, WS_SPLITK, (size_t)batch * splits * M * N * 4, &w));
+        p.part = (float*)w;
+        hipLaunchKernelGGL((gemm_bf16x6_dma_kernel<float, true>), grid, block, 0, h->stream, q);
+        if (tC == CT_F64)
+            splitk_reduce<double>(h->stream, p.part, (double*)C, M, N, splits, batch, ldc, bsC, alpha, beta, active);
+        else
+            splitk_reduce<float>(h->stream, p.part, (float*)C, M, N, splits, batch, ldc, bsC, alpha, beta, active);
+    } else if (tC == CT_F64) {
+        hipLaunchKernelGGL((gemm_bf16x6_dma_kernel<double, false>), grid, block, 0, h->stream, q);
+    } else {
+        hipLaunchKernelGGL((gemm_bf16x6_dma_kernel<float, false>), grid, block, 0, h->stream, q);
+    }
+    CT_HIP(hipGetLastError());
+    return 0;
+}
+
 // Generic entry used by ct_tensordot / the contractions when no pre-split
 // operand is available: split both operands into workspace, then run the NT
 // kernels.  A(m,k) and B(k,n) follow GemmDesc.
@@ -723,6 +1052,28 @@ int gemm_bf16x3(Handle* h, int tA, const void* A, int tB, const void* B, int tC,
     sa.p = (const uint16_t*)wa; sa.ld = ld; sa.bs = bsA;
     sb.p = (const uint16_t*)wb; sb.ld = ld; sb.bs = bsB;
     return gemm_bf16x3_nt(h, sa, sb, tC, C, g.M, g.N, g.K, g.ldc, g.bsC, g.batch, g.alpha, g.beta, g.active);
+}
+
+// The 6xBF16 form of gemm_bf16x3: both operands split with their l planes.
+int gemm_bf16x6(Handle* h, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g) {
+    const int64_t K32 = split_k32(g.K), ld = 2 * K32;
+    const int64_t bsA = (int64_t)g.M * ld, bsB = (int64_t)g.N * ld;
+    const int64_t bsAl = (int64_t)g.M * K32, bsBl = (int64_t)g.N * K32;
+    void *wa, *wb, *wal, *wbl;
+    CT_CHECK(ws_get(h, WS_CONV_A, (size_t)bsA * g.batch * 2, &wa));
+    CT_CHECK(ws_get(h, WS_CONV_B, (size_t)bsB * g.batch * 2, &wb));
+    CT_CHECK(ws_get(h, WS_CONV_AL, (size_t)bsAl * g.batch * 2, &wal));
+    CT_CHECK(ws_get(h, WS_CONV_BL, (size_t)bsBl * g.batch * 2, &wbl));
+    CT_CHECK(split_bf16(h, tA, A, g.M, g.K, g.sAm, g.sAk, g.batch, g.bsA, (uint16_t*)wa, ld, bsA, g.active,
+                        (uint16_t*)wal, K32, bsAl));
+    CT_CHECK(split_bf16(h, tB, B, g.N, g.K, g.sBn, g.sBk, g.batch, g.bsB, (uint16_t*)wb, ld, bsB, g.active,
+                        (uint16_t*)wbl, K32, bsBl));
+    SplitOp sa, sb;
+    sa.p = (const uint16_t*)wa; sa.ld = ld; sa.bs = bsA;
+    sa.l = (const uint16_t*)wal; sa.ldl = K32; sa.bsl = bsAl;
+    sb.p = (const uint16_t*)wb; sb.ld = ld; sb.bs = bsB;
+    sb.l = (const uint16_t*)wbl; sb.ldl = K32; sb.bsl = bsBl;
+    return gemm_bf16x6_nt(h, sa, sb, tC, C, g.M, g.N, g.K, g.ldc, g.bsC, g.batch, g.alpha, g.beta, g.active);
 }
 
 }  // namespace ct

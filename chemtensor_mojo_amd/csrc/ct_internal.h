@@ -70,8 +70,8 @@ struct Arena {
     char* base = nullptr;
     size_t cap = 0;
     size_t used = 0;
-    size_t off[40] = {};
-    size_t len[40] = {};
+    size_t off[48] = {};
+    size_t len[48] = {};
 };
 // Reserves `bytes` (256-byte aligned) for `slot` in the arena; false when the
 // arena is full.
@@ -83,7 +83,7 @@ struct Handle {
     bool own_stream = false;
     rocblas_handle blas = nullptr;
     // named workspaces (H_eff intermediates, reductions, SVD, Lanczos basis ...)
-    Buffer ws[40];
+    Buffer ws[48];
     Arena* arena = nullptr;       // set only for the duration of an allocation-free call
     double* d_scalars = nullptr;  // small FP64 scratch for reductions (pinned-size)
     double* h_pinned = nullptr;   // pinned host mirror for scalar read-back
@@ -133,7 +133,13 @@ enum WsSlot {
     WS_BND1 = 36,
     WS_BND2 = 37,
     WS_BND3 = 38,
-    WS_COUNT = 39
+    WS_LSPLIT_L = 39,  // l planes of the 6xBF16 operands: left environment,
+    WS_RSPLIT_L = 40,  // right environment,
+    WS_VSPLIT_L = 41,  // Krylov vector,
+    WS_USPLIT_L = 42,  // mixing output,
+    WS_CONV_AL = 43,   // generic operand conversions
+    WS_CONV_BL = 44,
+    WS_COUNT = 45
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);
@@ -168,24 +174,41 @@ size_t gemm_splitk_bytes(int prec, const GemmDesc& g);
 // for each 32-wide block kb of the contraction index it holds 32 hi = bf16(x)
 // values followed by the 32 lo = bf16(x - hi) values (one 128-byte line), with
 // k >= K zero-filled up to split_k32(K).  ld >= 2*split_k32(K).
+// A 6xBF16 operand adds the "l plane": element (row, k) at l + b*bsl + row*ldl + k
+// holds bf16(x - hi - lo), so that hi + lo + l carries every bit of the FP32
+// value; ldl >= split_k32(K), zero-filled like the split rows.
 struct SplitOp {
     const uint16_t* p = nullptr;
     int64_t ld = 0, bs = 0;
+    const uint16_t* l = nullptr;
+    int64_t ldl = 0, bsl = 0;
 };
 inline int64_t split_k32(int64_t K) { return (K + 31) / 32 * 32; }
 // Splits X (element (row, k) of batch b at X + b*bsX + row*s_row + k*s_k) into
 // split rows at out (ld_out, bs_out), zero-filling k in [K, split_k32(K)).
+// With outl, the l plane (ldl, bsl) is written as well.
 int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
-               int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active);
+               int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active,
+               uint16_t* outl = nullptr, int64_t ldl = 0, int64_t bsl = 0);
 // Same with rows contiguous (s_row == 1) and a two-level k: element k of a row
 // at X + (k / kin)*s_kc + (k % kin)*s_k.
 int split_bf16_kk(Handle* h, int tX, const void* X, int rows, int K, int kin, int64_t s_k, int64_t s_kc, int batch,
                   int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active);
-// Zeroes k in [K, split_k32(K)) of every row of a split-rows operand.
-int split_zero_pad(Handle* h, uint16_t* out, int64_t rows, int64_t K, int64_t ld);
+// Zeroes k in [K, split_k32(K)) of every row of a split-rows operand (and of
+// its l plane when outl is given).
+int split_zero_pad(Handle* h, uint16_t* out, int64_t rows, int64_t K, int64_t ld, uint16_t* outl = nullptr,
+                   int64_t ldl = 0);
 // C[b][m][n] = alpha * sum_k (Ah Bh + Ah Bl + Al Bh)[m,k][n,k] + beta * C
 int gemm_bf16x3_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* C, int M, int N, int K,
                    int64_t ldc, int64_t bsC, int batch, double alpha, double beta, const int* active);
+// 6xBF16 (both operands carry their l planes):
+// C = alpha * sum_k (Al Bh + Ah Bl + Am Bm + Am Bh + Ah Bm + Ah Bh) + beta * C,
+// with h, m, l = the split rows' hi, lo and the l plane.
+int gemm_bf16x6_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* C, int M, int N, int K,
+                   int64_t ldc, int64_t bsC, int batch, double alpha, double beta, const int* active);
+// Generic 3xBF16 / 6xBF16 GEMM on GemmDesc operands (splits both into workspace).
+int gemm_bf16x3(Handle* h, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g);
+int gemm_bf16x6(Handle* h, int tA, const void* A, int tB, const void* B, int tC, void* C, const GemmDesc& g);
 
 // ------------------------------------------------------------- mixing
 // Per pixel (p, q): Y[co, p, eo, q] (+)= sum_{ci, ei} coef[(ci*EIN+ei)*NOUT + co*EOUT+eo] * X[ci, p, ei, q]
@@ -209,8 +232,14 @@ size_t mix_pad_bytes(int K, int NO, int tAcc);
 // Output in split rows: m.sYp / m.sYe give the row offset (a multiple of the
 // row stride) of pixel p / output channel element eo, m.sYc the k offset of
 // output channel co; q adds to k.
+// With Yl, the l planes of a 6xBF16 operand are written too: the l value of
+// split-rows element (row offset o, k) goes to Yl[o / 2 + k], i.e. the l plane
+// has half the split rows' row stride.  Large MPO bonds are not supported with
+// Yl (the caller mixes in FP32 and splits).
 int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, const MixDesc& m,
-              const void* padded = nullptr);
+              const void* padded = nullptr, uint16_t* Yl = nullptr);
+// True when the mixing described by m takes the large-bond (GEMM) path.
+bool mix_desc_is_large(const MixDesc& m);
 
 // ------------------------------------------------------------- misc kernels
 int permute(Handle* h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm, void* out);
@@ -231,7 +260,8 @@ struct HeffDims {
 };
 
 // Environments prepared for one precision: FP64/FP32 keep plain pointers
-// (tEnv = element type); 3xBF16 holds the split, k-contiguous planes
+// (tEnv = element type); 3xBF16 / 6xBF16 hold the split, k-contiguous planes
+// (6xBF16 with their l planes)
 //   Ls: [wL][Dlb][Dl]  (A operand of L^T theta)
 //   Rs: [Drb][wR*Dr]   (B operand of U R)
 struct HeffEnv {
@@ -256,8 +286,8 @@ int heff_build_w2(Handle* h, int tW, const void* Wi, const void* Wj, int64_t wL,
 // Buffers owned by a prepared H_eff operator (ct_heff_create) instead of the
 // handle's workspaces, so the operator survives other calls on the handle.
 struct HeffStore {
-    Buffer w2, pad, l, r;
-    ~HeffStore() { buf_free(w2); buf_free(pad); buf_free(l); buf_free(r); }
+    Buffer w2, pad, l, r, ll, rl;
+    ~HeffStore() { buf_free(w2); buf_free(pad); buf_free(l); buf_free(r); buf_free(ll); buf_free(rl); }
 };
 int heff_prepare(Handle* h, int prec, int tEnv, const void* L, const void* R, const double* d_W2,
                  const HeffDims& hd, HeffEnv* env, HeffStore* store = nullptr);

@@ -26,6 +26,7 @@ struct MixArgs {
     int64_t P, Q;
     int64_t sXc, sXp, sXe, sYc, sYp, sYe;
     const int* active;
+    uint16_t* yl;  // OUT 3: l planes (half the split rows' row stride)
 };
 
 __device__ __forceinline__ uint16_t f2bf(float x) {
@@ -40,7 +41,8 @@ struct alignas(sizeof(T) * N) VecN {
     T v[N];
 };
 
-// OUT: 0 = double, 1 = float, 2 = split BF16 hi/lo planes (uint16).
+// OUT: 0 = double, 1 = float, 2 = split BF16 hi/lo planes (uint16), 3 = the
+// same plus the l planes of the 6xBF16 form (a.yl).
 // KM: compile-time size (16, 20, 32 or 64) of the zero-padded KM x KM coefficient
 // block `coef` (row k = input, column n = output).  Inputs k >= K re-read
 // input K-1 and meet zero coefficients, so the unrolled loops carry no
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void
         in_off[k] = ci * a.sXc + ei * a.sXe;
         const int nn = k < NO ? k : NO - 1;
         const int co = nn / a.EOUT, eo = nn - co * a.EOUT;
-        if (OUT == 2) {
+        if (OUT >= 2) {
             out_off[k] = eo * a.sYe;
             out_k[k] = co * a.sYc;
         } else {
@@ -117,15 +119,18 @@ __global__ __launch_bounds__(256) void mix_kernel(const TX* __restrict__ X, void
                 // (q .. q+VQ-1 stay inside one 32-wide block: Q % 32 == 0 when VQ > 1)
                 const int64_t k = out_k[n] + q;
                 const int64_t o = out_off[n] + p * a.sYp + 2 * (k & ~(int64_t)31) + (k & 31);
-                VecN<uint16_t, VQ> hv, lv;
+                VecN<uint16_t, VQ> hv, lv, mv;
 #pragma unroll
                 for (int j = 0; j < VQ; ++j) {
                     const float f = (float)acc[n][j];
                     hv.v[j] = f2bf(f);
-                    lv.v[j] = f2bf(f - __uint_as_float(((uint32_t)hv.v[j]) << 16));
+                    const float r = f - __uint_as_float(((uint32_t)hv.v[j]) << 16);
+                    lv.v[j] = f2bf(r);
+                    if (OUT == 3) mv.v[j] = f2bf(r - __uint_as_float(((uint32_t)lv.v[j]) << 16));
                 }
                 *(VecN<uint16_t, VQ>*)((uint16_t*)Yv + o) = hv;
                 *(VecN<uint16_t, VQ>*)((uint16_t*)Yv + o + 32) = lv;
+                if (OUT == 3) *(VecN<uint16_t, VQ>*)(a.yl + ((out_off[n] + p * a.sYp) >> 1) + k) = mv;
             }
         }
     }
@@ -251,6 +256,7 @@ int mix_args(const MixDesc& m, MixArgs* a) {
     a->sXc = m.sXc; a->sXp = m.sXp; a->sXe = m.sXe;
     a->sYc = m.sYc; a->sYp = m.sYp; a->sYe = m.sYe;
     a->active = m.active;
+    a->yl = nullptr;
     return 0;
 }
 
@@ -317,7 +323,7 @@ int launch_mix(Handle* h, const TX* X, void* Y, const double* d_coef, const MixA
     const auto div = [](int64_t v) { return v % VQV == 0; };
     const bool vec = VQV > 1 && KM <= 32 && NO == KM && div(a.Q) && div(a.sXc) && div(a.sXp) && div(a.sXe) && div(a.sYc) && div(a.sYp) &&
                      div(a.sYe) && (uintptr_t)X % 16 == 0 &&
-                     (OUT == 2 ? (a.Q % 32 == 0 && a.sYc % 32 == 0 && (uintptr_t)Y % (size_t)(VQV * 2) == 0)
+                     (OUT >= 2 ? (a.Q % 32 == 0 && a.sYc % 32 == 0 && (uintptr_t)Y % (size_t)(VQV * 2) == 0)
                                : (uintptr_t)Y % (size_t)(VQV * (OUT == 0 ? 8 : 4)) == 0);
     if (vec) return launch_mix_vq<TX, TACC, OUT, VQV>(h, X, Y, c, a, NO, KM);
     return launch_mix_vq<TX, TACC, OUT, 1>(h, X, Y, c, a, NO, KM);
@@ -503,34 +509,53 @@ int mix(Handle* h, int tX, const void* X, int tY, void* Y, const double* d_coef,
     return 0;
 }
 
-// Mixing whose output goes straight into 3xBF16 split rows; X is FP32.
+// Mixing whose output goes straight into 3xBF16 split rows (and, with Yl, the
+// l planes of the 6xBF16 form); X is FP32.
 int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, const MixDesc& m,
-              const void* padded) {
+              const void* padded, uint16_t* Yl) {
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
     CT_CHECK(mix_args(m, &a));
-    if (mix_is_large(a)) return mix_large_split(h, X, Y, d_coef, a);
-    CT_CHECK((launch_mix<float, float, 2>(h, X, (void*)Y, d_coef, a, padded)));
+    if (mix_is_large(a)) {
+        if (Yl) {
+            set_last_error("mix_split: the 6xBF16 output is not supported for large MPO bonds");
+            return CT_STATUS_NOT_SUPPORTED;
+        }
+        return mix_large_split(h, X, Y, d_coef, a);
+    }
+    if (Yl) {
+        a.yl = Yl;
+        CT_CHECK((launch_mix<float, float, 3>(h, X, (void*)Y, d_coef, a, padded)));
+    } else {
+        CT_CHECK((launch_mix<float, float, 2>(h, X, (void*)Y, d_coef, a, padded)));
+    }
     CT_HIP(hipGetLastError());
     return 0;
 }
 
+bool mix_desc_is_large(const MixDesc& m) {
+    MixArgs a;
+    mix_args(m, &a);
+    return mix_is_large(a);
+}
+
 __global__ void split_zero_pad_kernel(uint16_t* __restrict__ out, int64_t rows, int64_t K, int64_t npad,
-                                      int64_t ld) {
+                                      int64_t ld, uint16_t* __restrict__ outl, int64_t ldl) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= rows * npad) return;
     const int64_t r = e / npad, k = K + (e - r * npad);
     uint16_t* o = out + r * ld + 2 * (k & ~(int64_t)31) + (k & 31);
     o[0] = 0;
     o[32] = 0;
+    if (outl) outl[r * ldl + k] = 0;
 }
 
-int split_zero_pad(Handle* h, uint16_t* out, int64_t rows, int64_t K, int64_t ld) {
+int split_zero_pad(Handle* h, uint16_t* out, int64_t rows, int64_t K, int64_t ld, uint16_t* outl, int64_t ldl) {
     const int64_t npad = split_k32(K) - K;
     if (npad == 0 || rows <= 0) return 0;
     const int64_t n = rows * npad;
     hipLaunchKernelGGL(split_zero_pad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, out, rows,
-                       K, npad, ld);
+                       K, npad, ld, outl, ldl);
     CT_HIP(hipGetLastError());
     return 0;
 }

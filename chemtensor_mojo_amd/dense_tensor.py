@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import F32, F64, PREC_BF16X3, PREC_FP32, PREC_FP64, call
+from ._lib import F32, F64, PREC_BF16X3, PREC_BF16X6, PREC_FP32, PREC_FP64, call
 
 
 def _dtype_code(t: torch.dtype) -> int:
@@ -228,5 +228,5 @@ def dense_tensor_dot(C: DenseTensor, A: DenseTensor, B: DenseTensor, ctx: Device
 __all__ = [
     "DeviceContext", "DenseTensor", "dense_tensor_dot", "dot_product", "axpy_in_place",
     "create_dense_tensor", "create_dense_tensor_uninitialized", "create_dense_tensor_from_data",
-    "compute_row_major_strides", "PREC_FP64", "PREC_FP32", "PREC_BF16X3",
+    "compute_row_major_strides", "PREC_FP64", "PREC_FP32", "PREC_BF16X3", "PREC_BF16X6",
 ]

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import PREC_BF16X3, PREC_FP32, PREC_FP64, SVD_GESVD, SVD_GRAM, call
+from ._lib import PREC_BF16X3, PREC_BF16X6, PREC_FP32, PREC_FP64, SVD_GESVD, SVD_GRAM, call
 from .dense_tensor import DenseTensor, DeviceContext, default_prec
 from .state import MatrixProductOperator, MatrixProductState, MPOSite, MPSSite
 
@@ -24,7 +24,8 @@ from .state import MatrixProductOperator, MatrixProductState, MPOSite, MPSSite
 class DMRGParams:
     """DMRGParams (dmrg.mojo:22-36) plus precision controls without a reference counterpart.
 
-    prec            precision of the Krylov matvecs (PREC_FP64 / PREC_FP32 / PREC_BF16X3)
+    prec            precision of the Krylov matvecs (PREC_FP64 / PREC_FP32 / PREC_BF16X3 /
+                    PREC_BF16X6: FP32 accuracy from six BF16 products)
     refine          0: reference energies; 1: FP64 Rayleigh quotient of each local
                     Ritz vector; 2: plus one residual-correction step
     env_prec        precision of the environment updates
@@ -357,5 +358,5 @@ def dmrg_two_site(ctx: DeviceContext, mpo: MatrixProductOperator, mps: MatrixPro
 __all__ = [
     "DMRGParams", "DMRGStats", "DMRGRun", "dmrg_two_site", "apply_two_site_heff", "lanczos_two_site_optimize",
     "build_two_site_theta", "split_two_site_theta_svd", "dense_tensor_svd_trunc", "PREC_FP64", "PREC_FP32",
-    "PREC_BF16X3",
+    "PREC_BF16X3", "PREC_BF16X6",
 ]

@@ -33,7 +33,11 @@ extern "C" {
 enum {
     CT_PREC_FP64 = 0,   /* FP64 MFMA, FP64 accumulate (the oracle-equivalent path) */
     CT_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulate                              */
-    CT_PREC_BF16X3 = 2  /* a*b ~ ah*bh + ah*bl + al*bh on BF16 MFMA, FP32 accumulate */
+    CT_PREC_BF16X3 = 2, /* a*b ~ ah*bh + ah*bl + al*bh on BF16 MFMA, FP32 accumulate */
+    CT_PREC_BF16X6 = 3  /* FP32-accurate on BF16 MFMA: x = h + m + l (three bf16 values
+                         * carrying the FP32 significand), a*b ~ the six products with
+                         * terms down to 2^-16 |ab|, FP32 accumulate; no counterpart in
+                         * the reference (its arithmetic is FP32, which this matches) */
 };
 /* element type of buffers */
 enum { CT_F64 = 0, CT_F32 = 1 };
@@ -97,6 +101,18 @@ int ct_split_bf16(ct_handle_t h, int dtype, const void* X, int64_t rows, int64_t
                   uint16_t* out, int64_t ld);
 int ct_gemm_bf16x3_nt(ct_handle_t h, const uint16_t* A, int64_t lda, const uint16_t* B, int64_t ldb, int dtype_c,
                       void* C, int64_t M, int64_t N, int64_t K, int64_t ldc);
+/* 6xBF16 building blocks (CT_PREC_BF16X6; no reference counterpart).  An
+ * operand is the split rows above plus an "l plane": element (r, k) at
+ * outl[r*ldl + k] holds l = bf16(x - hi - lo), zero for k >= K up to
+ * roundup(K, 32); ldl >= roundup(K, 32), a multiple of 8.  ct_split_bf16x6
+ * writes both.  ct_gemm_bf16x6_nt computes C[m][n] = sum_k of the six products
+ * Al Bh + Ah Bl + Am Bm + Am Bh + Ah Bm + Ah Bh (h, m, l = hi, lo, l plane)
+ * with FP32 accumulation: FP32 GEMM accuracy on the BF16 matrix cores. */
+int ct_split_bf16x6(ct_handle_t h, int dtype, const void* X, int64_t rows, int64_t K, int64_t s_row, int64_t s_k,
+                    uint16_t* out, int64_t ld, uint16_t* outl, int64_t ldl);
+int ct_gemm_bf16x6_nt(ct_handle_t h, const uint16_t* A, int64_t lda, const uint16_t* Al, int64_t ldal,
+                      const uint16_t* B, int64_t ldb, const uint16_t* Bl, int64_t ldbl, int dtype_c, void* C,
+                      int64_t M, int64_t N, int64_t K, int64_t ldc);
 
 /* DenseTensor.transpose (dense_tensor.mojo:460-545): out axis i = in axis perm[i]; rank <= 8. */
 int ct_permute(ct_handle_t h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm,
