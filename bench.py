@@ -908,6 +908,7 @@ def main():
                             "rel_err": float(f"{gm['bf16x6']['rel_err']:.3g}"),
                             "rel_err_fp32": float(f"{gm['fp32']['rel_err']:.3g}"),
                             "vs_fp32_peak": round(gm["bf16x6"]["tflops"] / PEAK["fp32"], 4),
+                            "traffic": profiled_traffic("bf16x6_stage4_gemm", D, w),
                             "note": "FP32-accurate GEMM on the BF16 matrix cores: six BF16 products per "
                                     "multiply-add; peak = dense BF16 peak / 6, so frac is the MFMA issue "
                                     "fraction; rel_err against the FP64 product, beside the FP32 MFMA GEMM's"},
