@@ -168,7 +168,7 @@ def env_flops(D, d, w):
 
 
 def bench_env_theta(ctx, steps, warmup, D=1024, ws=(5, 8)):
-    """Left / right environment updates (FP64, FP32, 3xBF16 contractions on FP64 storage) and the two-site theta
+    """Left / right environment updates (FP64, FP32, 3xBF16, 6xBF16 contractions on FP64 storage) and the two-site theta
     build (FP64) at bond dimension D, seeded random site tensor and environment, XXZ (w=5) / J1-J2 (w=8) bulk
     MPO; mean of `steps` calls timed with HIP events."""
     import torch
@@ -185,7 +185,8 @@ def bench_env_theta(ctx, steps, warmup, D=1024, ws=(5, 8)):
         site, mpo = ct.MPSSite(A), ct.MPOSite(dev(W))
         F = env_flops(D, 2, w)
         row = {"gflop": round(F / 1e9, 3)}
-        for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3)):
+        for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3),
+                           ("bf16x6", ct.PREC_BF16X6)):
             tl, _ = time_events(lambda: ct.update_left_environment(E, site, mpo, ctx, prec=prec), steps, warmup)
             tr, _ = time_events(lambda: ct.update_right_environment(E, site, mpo, ctx, prec=prec), steps, warmup)
             row[name] = {"left_ms": round(tl * 1e3, 4), "left_tflops": round(F / tl / 1e12, 2),
@@ -214,7 +215,8 @@ def bench_matvec_grid(ctx, steps, warmup, Ds=(256, 512, 1024), ws=(5, 8)):
             tL, tR, tW, tth = dev(L), dev(R), ct.MPOSite(dev(W)), dev(th)
             F = matvec_flops(D, 2, D, w, w, w)
             row = {"gflop": round(F / 1e9, 3)}
-            for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3)):
+            for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3),
+                               ("bf16x6", ct.PREC_BF16X6)):
                 op = ct.HeffOperator(tL, tR, tW, tW, ctx, prec=prec)
                 buf = torch.empty(D * 4 * D, dtype=torch.float64, device="cuda")
                 t, _ = time_events(lambda: op.apply(tth, buf), steps, warmup)
@@ -222,7 +224,7 @@ def bench_matvec_grid(ctx, steps, warmup, Ds=(256, 512, 1024), ws=(5, 8)):
                 row[name + "_ms"] = round(t * 1e3, 4)
                 row[name + "_tflops"] = round(F / t / 1e12, 2)
             log(f"matvec grid w={w} D={D}: fp64 {row['fp64_tflops']} fp32 {row['fp32_tflops']} bf16x3 "
-                f"{row['bf16x3_tflops']} TFLOP/s")
+                f"{row['bf16x3_tflops']} bf16x6 {row['bf16x6_tflops']} TFLOP/s")
             grid[f"w{w}_D{D}"] = row
             del tL, tR, tth
             torch.cuda.empty_cache()
