@@ -131,7 +131,7 @@ def bench_matvec(ctx, D, w, steps, warmup):
 
 
 def bench_matvec_mpo(ctx, D, Wi, Wj, steps, warmup, seed=7):
-    """Prepared-operator matvec of given MPO tensors at bond dimension D (random symmetric environments), three
+    """Prepared-operator matvec of given MPO tensors at bond dimension D (random symmetric environments), four
     forms; returns {form: {"s_per_matvec", "tflops"}} and the relative errors against FP64."""
     import torch
     import chemtensor_mojo_amd as ct
@@ -147,14 +147,15 @@ def bench_matvec_mpo(ctx, D, Wi, Wj, steps, warmup, seed=7):
     tL, tR, tth = dev(L), dev(R), dev(th)
     F = matvec_flops(D, 2, D, wL, Wi.shape[3], wR)
     res, outs = {}, {}
-    for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3)):
+    for name, prec in (("fp64", ct.PREC_FP64), ("fp32", ct.PREC_FP32), ("bf16x3", ct.PREC_BF16X3),
+                       ("bf16x6", ct.PREC_BF16X6)):
         op = ct.HeffOperator(tL, tR, ct.MPOSite(dev(Wi)), ct.MPOSite(dev(Wj)), ctx, prec=prec)
         buf = torch.empty(D * 4 * D, dtype=torch.float64, device="cuda")
         t, _ = time_events(lambda: op.apply(tth, buf), steps, warmup)
         outs[name] = buf.cpu().numpy()
         op.close()
         res[name] = {"s_per_matvec": t, "tflops": F / t / 1e12}
-    for name in ("fp32", "bf16x3"):
+    for name in ("fp32", "bf16x3", "bf16x6"):
         res[name]["rel_err_vs_fp64"] = float(np.linalg.norm(outs[name] - outs["fp64"]) / np.linalg.norm(outs["fp64"]))
     del tL, tR, tth
     torch.cuda.empty_cache()
@@ -627,7 +628,7 @@ def study_qc16(ctx, args):
     particle-number penalty 1.0 (N - 16)^2, exact MPO from the operator strings (central bond dimension 562), seeded
     random start D=16 inside the 16-electron sector (hamiltonians.number_sector_random_mps_tensors; H conserves N, so
     the sweeps stay there), chi_max = 512, eps_trunc = 1e-12, 6 sweeps, 20 Krylov vectors, exact tridiagonal solves;
-    FP64 against FP32 and 3xBF16 with FP64 refinement.  Then one more sweep in each form, and in FP64 with the same
+    FP64 against FP32, 3xBF16 and 6xBF16 with FP64 refinement.  Then one more sweep in each form, and in FP64 with the same
     refinement, from the same FP64 state (isolates the arithmetic from the different trajectories of truncated runs),
     and <(N - 16)^2> of each result."""
     import chemtensor_mojo_amd as ct
@@ -644,7 +645,8 @@ def study_qc16(ctx, args):
         log(f"center-bond matvec {k}: {r['s_per_matvec'] * 1e3:.2f} ms {r['tflops']:.1f} TFLOP/s "
             f"(err vs fp64 {r.get('rel_err_vs_fp64', 0.0):.1e})")
     mpo = ct.mpo_from_tensors(ctx, mpo_t)
-    forms = (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2), ("bf16x3_refined", ct.PREC_BF16X3, 2))
+    forms = (("fp64", ct.PREC_FP64, 0), ("fp32_refined", ct.PREC_FP32, 2), ("bf16x3_refined", ct.PREC_BF16X3, 2),
+             ("bf16x6_refined", ct.PREC_BF16X6, 2))
 
     def params(prec, refine, krylov=20):
         return ct.DMRGParams(num_sweeps=1, chi_max=D, eps_trunc=1e-12, max_krylov_iter=krylov, krylov_tol=1e-12,
@@ -694,7 +696,7 @@ def study_qc16(ctx, args):
 
 def study_qc16_matvec(ctx, args):
     """The H_eff matvec at the centre bond of the 16-orbital quantum-chemistry MPO (w = 501, 562, 501; d = 2), D = 512,
-    random symmetric environments, prepared operator, FP64 / FP32 / 3xBF16 (`--steps` timed calls each); used for
+    random symmetric environments, prepared operator, FP64 / FP32 / 3xBF16 / 6xBF16 (`--steps` timed calls each); used for
     the kernel profile of the large-w path."""
     from chemtensor_mojo_amd import hamiltonians as ham
     h, v = ham.qc_integrals(16, 42)

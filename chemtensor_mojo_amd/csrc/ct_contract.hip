@@ -206,16 +206,7 @@ int heff_apply(Handle* h, const HeffEnv& env, int tVec, const void* theta, void*
         MixDesc ms = m;
         ms.sYc = D.Dr; ms.sYp = d2 * ldU; ms.sYe = ldU;
         CT_CHECK(split_zero_pad(h, uh, D.Dlb * d2, KU, ldU, (uint16_t*)ul, KU32));
-        if (x6 && mix_desc_is_large(ms)) {
-            // large MPO bonds: FP32 mixing (the batched-GEMM path), then the 6xBF16 split of U
-            void* u32;
-            CT_CHECK(ws_get(h, WS_T2, (size_t)nU * 4, &u32));
-            CT_CHECK(mix(h, CT_F32, t1, CT_F32, u32, env.W2, m));
-            CT_CHECK(split_bf16(h, CT_F32, u32, (int)(D.Dlb * d2), (int)KU, KU, 1, 1, 0, uh, ldU, 0, active,
-                                (uint16_t*)ul, KU32, 0));
-        } else {
-            CT_CHECK(mix_split(h, (const float*)t1, uh, env.W2, ms, env.W2pad, (uint16_t*)ul));
-        }
+        CT_CHECK(mix_split(h, (const float*)t1, uh, env.W2, ms, env.W2pad, (uint16_t*)ul));
         SplitOp su;
         su.p = uh; su.ld = ldU; su.bs = 0;
         su.l = (const uint16_t*)ul; su.ldl = KU32; su.bsl = 0;

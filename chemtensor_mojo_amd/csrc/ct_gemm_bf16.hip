@@ -847,20 +847,22 @@ int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row,
 }
 
 int split_bf16_kk(Handle* h, int tX, const void* X, int rows, int K, int kin, int64_t s_k, int64_t s_kc, int batch,
-                  int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active) {
+                  int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active, uint16_t* outl,
+                  int64_t ldl, int64_t bsl) {
     if (rows <= 0 || K <= 0 || batch <= 0) return 0;
-    if (kin < 1 || ld_out % 8 || bs_out % 8 || ((uintptr_t)out % 16) || ld_out < 2 * split_k32(K)) {
-        set_last_error("split_bf16_kk: needs kin >= 1, ld >= 2*roundup(K,32), ld and batch stride multiples of 8, "
-                       "a 16-byte aligned output");
+    if (kin < 1 || ld_out % 8 || bs_out % 8 || ((uintptr_t)out % 16) || ld_out < 2 * split_k32(K) ||
+        (outl && (ldl % 8 || bsl % 8 || ((uintptr_t)outl % 16) || ldl < split_k32(K)))) {
+        set_last_error("split_bf16_kk: needs kin >= 1, ld >= 2*roundup(K,32) (l plane: ldl >= roundup(K,32)), "
+                       "strides multiples of 8, 16-byte aligned outputs");
         return CT_STATUS_SHAPE;
     }
     dim3 grid((unsigned)((rows + 63) / 64), (unsigned)(split_k32(K) / 32), (unsigned)batch), block(256);
     if (tX == CT_F64)
         hipLaunchKernelGGL((split_t_kernel<double, true>), grid, block, 0, h->stream, (const double*)X, rows, K, kin,
-                           s_k, s_kc, bsX, out, ld_out, bs_out, active, (uint16_t*)nullptr, (int64_t)0, (int64_t)0);
+                           s_k, s_kc, bsX, out, ld_out, bs_out, active, outl, ldl, bsl);
     else
         hipLaunchKernelGGL((split_t_kernel<float, true>), grid, block, 0, h->stream, (const float*)X, rows, K, kin,
-                           s_k, s_kc, bsX, out, ld_out, bs_out, active, (uint16_t*)nullptr, (int64_t)0, (int64_t)0);
+                           s_k, s_kc, bsX, out, ld_out, bs_out, active, outl, ldl, bsl);
     CT_HIP(hipGetLastError());
     return 0;
 }

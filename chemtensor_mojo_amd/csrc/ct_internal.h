@@ -139,7 +139,9 @@ enum WsSlot {
     WS_USPLIT_L = 42,  // mixing output,
     WS_CONV_AL = 43,   // generic operand conversions
     WS_CONV_BL = 44,
-    WS_COUNT = 45
+    WS_MIXA_L = 45,    // large-w mixing, 6xBF16: l planes of the split input
+    WS_MIXC_L = 46,    // and of the split coefficients
+    WS_COUNT = 47
 };
 
 int ws_get(Handle* h, int slot, size_t bytes, void** out);
@@ -193,7 +195,8 @@ int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row,
 // Same with rows contiguous (s_row == 1) and a two-level k: element k of a row
 // at X + (k / kin)*s_kc + (k % kin)*s_k.
 int split_bf16_kk(Handle* h, int tX, const void* X, int rows, int K, int kin, int64_t s_k, int64_t s_kc, int batch,
-                  int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active);
+                  int64_t bsX, uint16_t* out, int64_t ld_out, int64_t bs_out, const int* active,
+                  uint16_t* outl = nullptr, int64_t ldl = 0, int64_t bsl = 0);
 // Zeroes k in [K, split_k32(K)) of every row of a split-rows operand (and of
 // its l plane when outl is given).
 int split_zero_pad(Handle* h, uint16_t* out, int64_t rows, int64_t K, int64_t ld, uint16_t* outl = nullptr,
@@ -234,12 +237,9 @@ size_t mix_pad_bytes(int K, int NO, int tAcc);
 // output channel co; q adds to k.
 // With Yl, the l planes of a 6xBF16 operand are written too: the l value of
 // split-rows element (row offset o, k) goes to Yl[o / 2 + k], i.e. the l plane
-// has half the split rows' row stride.  Large MPO bonds are not supported with
-// Yl (the caller mixes in FP32 and splits).
+// has half the split rows' row stride.
 int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, const MixDesc& m,
               const void* padded = nullptr, uint16_t* Yl = nullptr);
-// True when the mixing described by m takes the large-bond (GEMM) path.
-bool mix_desc_is_large(const MixDesc& m);
 
 // ------------------------------------------------------------- misc kernels
 int permute(Handle* h, int dtype, const void* in, int rank, const int64_t* shape, const int* perm, void* out);

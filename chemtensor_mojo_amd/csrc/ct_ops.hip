@@ -366,13 +366,17 @@ __global__ void mix_unpack_kernel(const TT* __restrict__ Yt, void* __restrict__ 
         } else if (OUT == 1) {
             ((float*)Yv)[co * a.sYc + p * a.sYp + eo * a.sYe + q] = (float)v;
         } else {
+            // split rows (OUT 2), plus the l plane at half the row stride (OUT 3)
             const int64_t k = co * a.sYc + q;
-            const int64_t o = eo * a.sYe + p * a.sYp + 2 * (k & ~(int64_t)31) + (k & 31);
+            const int64_t row = eo * a.sYe + p * a.sYp;
+            const int64_t o = row + 2 * (k & ~(int64_t)31) + (k & 31);
             const float f = (float)v;
             const uint16_t hb = f2bf(f);
-            const float hf = __uint_as_float(((uint32_t)hb) << 16);
+            const float r = f - __uint_as_float(((uint32_t)hb) << 16);
+            const uint16_t lb = f2bf(r);
             ((uint16_t*)Yv)[o] = hb;
-            ((uint16_t*)Yv)[o + 32] = f2bf(f - hf);
+            ((uint16_t*)Yv)[o + 32] = lb;
+            if (OUT == 3) a.yl[(row >> 1) + k] = f2bf(r - __uint_as_float(((uint32_t)lb) << 16));
         }
     }
 }
@@ -380,6 +384,7 @@ __global__ void mix_unpack_kernel(const TT* __restrict__ Yt, void* __restrict__ 
 // Split-rows unpack with four neighbouring q per thread (Q, sYc, sYe, sYp
 // multiples of 4): each thread writes its four hi and four lo BF16 values as
 // one 8-byte store each.
+template <bool L3>
 __global__ void mix_unpack_split4_kernel(const float* __restrict__ Yt, uint16_t* __restrict__ Y, MixArgs a) {
     if (a.active && *a.active == 0) return;
     const int NO = a.COUT * a.EOUT;
@@ -392,17 +397,23 @@ __global__ void mix_unpack_split4_kernel(const float* __restrict__ Yt, uint16_t*
         const int co = n / a.EOUT, eo = n - co * a.EOUT;
         const float4 v = *(const float4*)(Yt + pn * a.Q + q);
         const int64_t k = co * a.sYc + q;  // k % 4 == 0: the four k share a 32-block
-        const int64_t o = eo * a.sYe + p * a.sYp + 2 * (k & ~(int64_t)31) + (k & 31);
+        const int64_t row = eo * a.sYe + p * a.sYp;
+        const int64_t o = row + 2 * (k & ~(int64_t)31) + (k & 31);
         const float f[4] = {v.x, v.y, v.z, v.w};
-        uint16_t hi[4], lo[4];
+        uint16_t hi[4], lo[4], l3[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             hi[i] = f2bf(f[i]);
-            lo[i] = f2bf(f[i] - __uint_as_float(((uint32_t)hi[i]) << 16));
+            const float r = f[i] - __uint_as_float(((uint32_t)hi[i]) << 16);
+            lo[i] = f2bf(r);
+            l3[i] = f2bf(r - __uint_as_float(((uint32_t)lo[i]) << 16));
         }
         *(uint2*)(Y + o) = make_uint2((uint32_t)hi[0] | ((uint32_t)hi[1] << 16), (uint32_t)hi[2] | ((uint32_t)hi[3] << 16));
         *(uint2*)(Y + o + 32) =
             make_uint2((uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16));
+        if (L3)  // the l plane: half the row stride, the same k
+            *(uint2*)(a.yl + (row >> 1) + k) =
+                make_uint2((uint32_t)l3[0] | ((uint32_t)l3[1] << 16), (uint32_t)l3[2] | ((uint32_t)l3[3] << 16));
     }
 }
 
@@ -430,30 +441,56 @@ int mix_large(Handle* h, const TX* X, void* Y, const double* d_coef, const MixAr
     return 0;
 }
 
-// 3xBF16 form with split-rows output: the coefficients (rows n) and X (rows q
-// of each pixel row p, k = (ci, ei)) are split once into bf16 hi/lo rows and
-// the three-product NT GEMM forms Yt[p][n][q] in FP32 before the unpack.
-int mix_large_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, const MixArgs& a) {
+// 3xBF16 / 6xBF16 forms with split-rows output: the coefficients (rows n) and
+// X (rows q of each pixel row p, k = (ci, ei)) are split once into bf16 rows
+// (6xBF16: with their l planes) and the three- or six-product NT GEMM forms
+// Yt[p][n][q] in FP32 before the unpack (6xBF16: also into the l planes, Yl).
+int mix_large_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, const MixArgs& a, uint16_t* Yl) {
     const int K = a.CIN * a.EIN, NO = a.COUT * a.EOUT;
-    const int64_t ld = 2 * split_k32(K);
-    void *wx, *wy, *wc;
+    const int64_t K32 = split_k32(K), ld = 2 * K32;
+    void *wx, *wy, *wc, *wxl = nullptr, *wcl = nullptr;
     CT_CHECK(ws_get(h, WS_MIXA, (size_t)(a.P * a.Q * ld) * 2, &wx));
     CT_CHECK(ws_get(h, WS_MIXB, (size_t)(a.P * NO * a.Q) * 4, &wy));
     CT_CHECK(ws_get(h, WS_MIXC, (size_t)(NO * ld) * 2, &wc));
-    CT_CHECK(split_bf16(h, CT_F64, d_coef, NO, K, 1, NO, 1, 0, (uint16_t*)wc, ld, 0, a.active));
+    if (Yl) {
+        CT_CHECK(ws_get(h, WS_MIXA_L, (size_t)(a.P * a.Q * K32) * 2, &wxl));
+        CT_CHECK(ws_get(h, WS_MIXC_L, (size_t)(NO * K32) * 2, &wcl));
+    }
+    CT_CHECK(split_bf16(h, CT_F64, d_coef, NO, K, 1, NO, 1, 0, (uint16_t*)wc, ld, 0, a.active, (uint16_t*)wcl, K32,
+                        0));
     CT_CHECK(split_bf16_kk(h, CT_F32, X, (int)a.Q, K, a.EIN, a.sXe, a.sXc, (int)a.P, a.sXp, (uint16_t*)wx, ld,
-                           a.Q * ld, a.active));
-    const SplitOp A{(const uint16_t*)wc, ld, 0}, B{(const uint16_t*)wx, ld, a.Q * ld};
-    CT_CHECK(gemm_bf16x3_nt(h, A, B, CT_F32, wy, NO, (int)a.Q, K, a.Q, (int64_t)NO * a.Q, (int)a.P, 1.0, 0.0,
-                            a.active));
+                           a.Q * ld, a.active, (uint16_t*)wxl, K32, a.Q * K32));
+    SplitOp A, B;
+    A.p = (const uint16_t*)wc; A.ld = ld; A.bs = 0;
+    B.p = (const uint16_t*)wx; B.ld = ld; B.bs = a.Q * ld;
+    if (Yl) {
+        A.l = (const uint16_t*)wcl; A.ldl = K32; A.bsl = 0;
+        B.l = (const uint16_t*)wxl; B.ldl = K32; B.bsl = a.Q * K32;
+        CT_CHECK(gemm_bf16x6_nt(h, A, B, CT_F32, wy, NO, (int)a.Q, K, a.Q, (int64_t)NO * a.Q, (int)a.P, 1.0, 0.0,
+                                a.active));
+    } else {
+        CT_CHECK(gemm_bf16x3_nt(h, A, B, CT_F32, wy, NO, (int)a.Q, K, a.Q, (int64_t)NO * a.Q, (int)a.P, 1.0, 0.0,
+                                a.active));
+    }
+    MixArgs au = a;
+    au.yl = Yl;
     const int64_t nout = a.P * (int64_t)NO * a.Q;
-    if (a.Q % 4 == 0 && a.sYc % 4 == 0 && a.sYe % 4 == 0 && a.sYp % 4 == 0 && ((uintptr_t)Y % 8) == 0)
-        hipLaunchKernelGGL(mix_unpack_split4_kernel,
-                           dim3((unsigned)std::min<int64_t>((nout / 4 + 255) / 256, 16384)), dim3(256), 0, h->stream,
-                           (const float*)wy, Y, a);
-    else
-        hipLaunchKernelGGL((mix_unpack_kernel<float, 2>), dim3((unsigned)std::min<int64_t>((nout + 255) / 256, 16384)),
-                           dim3(256), 0, h->stream, (const float*)wy, (void*)Y, a);
+    if (a.Q % 4 == 0 && a.sYc % 4 == 0 && a.sYe % 4 == 0 && a.sYp % 4 == 0 && ((uintptr_t)Y % 8) == 0 &&
+        ((uintptr_t)Yl % 8) == 0) {
+        const dim3 grid((unsigned)std::min<int64_t>((nout / 4 + 255) / 256, 16384));
+        if (Yl)
+            hipLaunchKernelGGL(mix_unpack_split4_kernel<true>, grid, dim3(256), 0, h->stream, (const float*)wy, Y, au);
+        else
+            hipLaunchKernelGGL(mix_unpack_split4_kernel<false>, grid, dim3(256), 0, h->stream, (const float*)wy, Y, au);
+    } else {
+        const dim3 grid((unsigned)std::min<int64_t>((nout + 255) / 256, 16384));
+        if (Yl)
+            hipLaunchKernelGGL((mix_unpack_kernel<float, 3>), grid, dim3(256), 0, h->stream, (const float*)wy, (void*)Y,
+                               au);
+        else
+            hipLaunchKernelGGL((mix_unpack_kernel<float, 2>), grid, dim3(256), 0, h->stream, (const float*)wy, (void*)Y,
+                               au);
+    }
     CT_HIP(hipGetLastError());
     return 0;
 }
@@ -516,13 +553,7 @@ int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, cons
     if (m.P <= 0 || m.Q <= 0) return 0;
     MixArgs a;
     CT_CHECK(mix_args(m, &a));
-    if (mix_is_large(a)) {
-        if (Yl) {
-            set_last_error("mix_split: the 6xBF16 output is not supported for large MPO bonds");
-            return CT_STATUS_NOT_SUPPORTED;
-        }
-        return mix_large_split(h, X, Y, d_coef, a);
-    }
+    if (mix_is_large(a)) return mix_large_split(h, X, Y, d_coef, a, Yl);
     if (Yl) {
         a.yl = Yl;
         CT_CHECK((launch_mix<float, float, 3>(h, X, (void*)Y, d_coef, a, padded)));
@@ -531,12 +562,6 @@ int mix_split(Handle* h, const float* X, uint16_t* Y, const double* d_coef, cons
     }
     CT_HIP(hipGetLastError());
     return 0;
-}
-
-bool mix_desc_is_large(const MixDesc& m) {
-    MixArgs a;
-    mix_args(m, &a);
-    return mix_is_large(a);
 }
 
 __global__ void split_zero_pad_kernel(uint16_t* __restrict__ out, int64_t rows, int64_t K, int64_t npad,

@@ -166,8 +166,10 @@ def _rand_problem(rng, Dl, Dr, d, wL, wM, wR):
 
 @pytest.mark.parametrize("shape", [(1, 2, 2, 1, 5, 1), (7, 9, 2, 5, 5, 5), (64, 64, 2, 5, 5, 5),
                                    (130, 96, 2, 8, 8, 8), (256, 256, 2, 8, 8, 8), (40, 40, 3, 4, 4, 4),
-                                   # w*d^2 > 64: FP32 mixing on the batched-GEMM path, then the 6xBF16 split
-                                   (64, 48, 2, 40, 40, 40), (24, 20, 2, 501, 562, 501)])
+                                   # w*d^2 > 64: the mixing as a batched six-product GEMM (quantum-chemistry
+                                   # MPO bonds), with the 8-byte and the per-element unpack
+                                   (64, 48, 2, 40, 40, 40), (24, 20, 2, 501, 562, 501), (20, 9, 2, 40, 40, 40),
+                                   (64, 24, 2, 501, 562, 501)])
 def test_heff_bf16x6_vs_oracle(ctx, ora, shape):
     """H_eff in the 6xBF16 form against the oracle, FP64 and FP32 storage, to
     the FP32 form's tolerance."""

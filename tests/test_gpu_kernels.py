@@ -589,7 +589,7 @@ def test_dmrg_heisenberg_afm_vs_oracle_and_ed(ctx, ora, svd_mode):
 
 # ----------------------------------------------------- quantum chemistry
 @pytest.mark.parametrize("prec,refine,tol", [(ct.PREC_FP64, 0, 1e-10), (ct.PREC_FP32, 2, 1e-9),
-                                             (ct.PREC_BF16X3, 2, 1e-9)])
+                                             (ct.PREC_BF16X3, 2, 1e-9), (ct.PREC_BF16X6, 2, 1e-9)])
 def test_dmrg_quantum_chemistry_vs_fci(ctx, ora, prec, refine, tol):
     """4 spatial orbitals (8 spin-orbital sites), random integrals, half filling
     through a particle-number penalty, exact MPO (bond dimensions up to 46, so
