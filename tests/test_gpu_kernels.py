@@ -629,7 +629,8 @@ def test_dmrg_quantum_chemistry_number_sector_start(ctx, prec, refine):
 
 @pytest.mark.parametrize("prec,refine,max_rot", [(ct.PREC_FP64, 0, -1), (ct.PREC_BF16X3, 2, -1),
                                                  (ct.PREC_FP32, 0, -1), (ct.PREC_FP32, 2, -1),
-                                                 (ct.PREC_FP64, 1, -1), (ct.PREC_FP64, 0, 200)])
+                                                 (ct.PREC_FP64, 1, -1), (ct.PREC_FP64, 0, 200),
+                                                 (ct.PREC_BF16X6, 2, -1)])
 def test_dmrg_graph_replay_matches_eager(ctx, prec, refine, max_rot):
     """Each bond's Lanczos solve replayed from its cached HIP graph gives the same
     sweeps, bit for bit, as launching every kernel.  The bond dimension is fixed
