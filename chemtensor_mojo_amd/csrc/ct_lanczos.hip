@@ -589,7 +589,10 @@ int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double*
     LzState* s = (LzState*)sb;
     const int* act = &s->active;
     const double thr = 100.0 * (double)dim * 2.2204460492503131e-16;
-    const double floor_rel = env.prec == CT_PREC_FP64 ? 0.0 : 1e-5;
+    // Noise floor of the mixed-precision recurrences (beta_j < floor_rel * max|alpha|
+    // ends the Krylov space), about 2-10x the relative matvec error of the form:
+    // FP32 and 3xBF16 1e-5 (matvec errors 1e-6 - 8e-6), 6xBF16 1e-6 (3e-7 - 1.2e-6).
+    const double floor_rel = env.prec == CT_PREC_FP64 ? 0.0 : env.prec == CT_PREC_BF16X6 ? 1e-6 : 1e-5;
     const bool exact_start = env.prec != CT_PREC_FP64 && refine >= 1 && env64 != nullptr;
     hipStream_t st = h->stream;
     const int G = grid_n(dim);
