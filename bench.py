@@ -769,14 +769,23 @@ def cpu_baseline(D, w, budget_s):
 
 
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it, with its family / model / stepping
+    numbers (the model name alone can be generic on virtualised hosts) and the
+    logical CPUs the process may use."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for ln in f:
-                if ln.startswith("model name"):
-                    return ln.split(":", 1)[1].strip()
+                if not ln.strip():
+                    break  # the first processor's block is enough
+                k, _, v = ln.partition(":")
+                fields[k.strip()] = v.strip()
     except OSError:
         pass
-    return "unknown"
+    name = fields.get("model name", "unknown")
+    ids = ", ".join(f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields)
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return f"{name} ({ids}; {ncpu} logical CPUs available)" if ids else f"{name} ({ncpu} logical CPUs available)"
 
 
 def profiled_traffic(key, D, w):
