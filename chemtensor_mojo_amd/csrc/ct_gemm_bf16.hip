@@ -813,6 +813,155 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(NtArgs p) {
         }
 }
 
+// 6xBF16 on 128x128x32 blocks (problems too small to give every CU a 256x128
+// tile): 4 waves (2 x 2) of 64 x 64, operands staged through registers into a
+// double-buffered LDS (h, m and l planes of both tiles, 120 KiB), h*H and the
+// five smaller products in separate accumulators as in the 256x128 kernel.
+// Thread t copies the 16 h, 16 m and 16 l values of k-half t / 128 of row
+// t % 128 (six 16-byte loads); rows past the edge re-read the last row.
+__device__ __forceinline__ void nt6_load(const uint16_t* P, int64_t ld, const uint16_t* Pl, int64_t ldl, int R0,
+                                         int RN, int k0, v4u* r) {
+    const int row = min(R0 + (int)(threadIdx.x & 127), RN - 1), kh = threadIdx.x >> 7;
+    const v4u* q = (const v4u*)(P + (int64_t)row * ld + 2 * k0 + 16 * kh);
+    const v4u* ql = (const v4u*)(Pl + (int64_t)row * ldl + k0 + 16 * kh);
+    r[0] = q[0];
+    r[1] = q[1];
+    r[2] = q[4];  // lo values sit 32 elements (64 B) after the hi ones
+    r[3] = q[5];
+    r[4] = ql[0];
+    r[5] = ql[1];
+}
+
+__device__ __forceinline__ void nt6_store(uint16_t (*Sh)[LDK], uint16_t (*Sm)[LDK], uint16_t (*Sl)[LDK],
+                                          const v4u* r) {
+    const int row = threadIdx.x & 127, kh = threadIdx.x >> 7;
+    v4u* h = (v4u*)&Sh[row][16 * kh];
+    v4u* m = (v4u*)&Sm[row][16 * kh];
+    v4u* l = (v4u*)&Sl[row][16 * kh];
+    h[0] = r[0];
+    h[1] = r[1];
+    m[0] = r[2];
+    m[1] = r[3];
+    l[0] = r[4];
+    l[1] = r[5];
+}
+
+template <typename TO>
+__global__ __launch_bounds__(256) void gemm_bf16x6_nt_kernel(X6Args q) {
+    const NtArgs& p = q.nt;
+    if (p.active && *p.active == 0) return;
+    __shared__ __attribute__((aligned(16))) uint16_t sAh[2][SM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sAm[2][SM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sAl[2][SM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBh[2][SM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBm[2][SM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t sBl[2][SM][LDK];
+
+    int tm, tn;
+    gemm_tile_coords(blockIdx.x, gridDim.x, p.tilesM, p.tilesN, tm, tn);
+    const int m0 = tm * SM, n0 = tn * SM;
+    const int zz = blockIdx.z;  // batch * splits + split
+    const int z = zz / p.splits;
+    const int kbeg = (zz - z * p.splits) * p.kchunk;
+    const int kend = min(p.K32, kbeg + p.kchunk);
+    const uint16_t* A = p.A + (int64_t)z * p.bsA;
+    const uint16_t* B = p.B + (int64_t)z * p.bsB;
+    const uint16_t* Al = q.Al + (int64_t)z * q.bsAl;
+    const uint16_t* Bl = q.Bl + (int64_t)z * q.bsBl;
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int li = lane & 15, lk = lane >> 4;
+
+    v4f acc[4][4], acs[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+            acs[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        }
+
+    v4u ra[6], rb[6];
+    if (kbeg < kend) {
+        nt6_load(A, p.lda, Al, q.ldal, m0, p.M, kbeg, ra);
+        nt6_load(B, p.ldb, Bl, q.ldbl, n0, p.N, kbeg, rb);
+        nt6_store(sAh[0], sAm[0], sAl[0], ra);
+        nt6_store(sBh[0], sBm[0], sBl[0], rb);
+    }
+    __syncthreads();
+
+    int buf = 0;
+    for (int k0 = kbeg; k0 < kend; k0 += BK) {
+        const bool more = k0 + BK < kend;
+        if (more) {
+            nt6_load(A, p.lda, Al, q.ldal, m0, p.M, k0 + BK, ra);
+            nt6_load(B, p.ldb, Bl, q.ldbl, n0, p.N, k0 + BK, rb);
+        }
+        v8bf ah[4], am[4], al[4], bh[4], bm[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = wm * 64 + i * 16 + li;
+            ah[i] = __builtin_bit_cast(v8bf, *(const v4u*)&sAh[buf][row][8 * lk]);
+            am[i] = __builtin_bit_cast(v8bf, *(const v4u*)&sAm[buf][row][8 * lk]);
+            al[i] = __builtin_bit_cast(v8bf, *(const v4u*)&sAl[buf][row][8 * lk]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = wn * 64 + j * 16 + li;
+            bh[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBh[buf][col][8 * lk]);
+            bm[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBm[buf][col][8 * lk]);
+            bl[j] = __builtin_bit_cast(v8bf, *(const v4u*)&sBl[buf][col][8 * lk]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acs[i][j], 0, 0, 0);
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acs[i][j], 0, 0, 0);
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[i], bm[j], acs[i][j], 0, 0, 0);
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am[i], bh[j], acs[i][j], 0, 0, 0);
+                acs[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bm[j], acs[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            }
+        if (more) {
+            nt6_store(sAh[buf ^ 1], sAm[buf ^ 1], sAl[buf ^ 1], ra);
+            nt6_store(sBh[buf ^ 1], sBm[buf ^ 1], sBl[buf ^ 1], rb);
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += acs[i][j];
+    if (p.splits > 1) {  // FP32 partial sums of this K range, reduced by splitk_reduce
+        float* P = p.part + (int64_t)zz * p.M * p.N;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = n0 + wn * 64 + j * 16 + li;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = m0 + wm * 64 + i * 16 + lk * 4 + r;
+                    if (m < p.M && n < p.N) P[(int64_t)m * p.N + n] = acc[i][j][r];
+                }
+            }
+        return;
+    }
+    TO* C = (TO*)p.C + (int64_t)z * p.bsC;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int n = n0 + wn * 64 + j * 16 + li;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) store_out(p, C, m0 + wm * 64 + i * 16 + lk * 4 + r, n, acc[i][j][r]);
+        }
+}
+
 }  // namespace
 
 int split_bf16(Handle* h, int tX, const void* X, int rows, int K, int64_t s_row, int64_t s_k, int batch,
@@ -1002,6 +1151,39 @@ int gemm_bf16x6_nt(Handle* h, const SplitOp& A, const SplitOp& B, int tC, void* 
     const int64_t tiles = (int64_t)p.tilesM * p.tilesN * batch;
     int splits = 1;
     while (tiles * splits < 256 && K32 / (splits * 2) >= 512) splits *= 2;
+    if (tiles * splits < 128) {
+        // fewer 256x128 tiles than half the CUs even after splitting K: the
+        // 128x128 tiling, split over K down to 4 K blocks per split (faster
+        // below this point, slower above it: 1.6x at 4096 x 1024 x 8192)
+        p.tilesM = (M + SM - 1) / SM;
+        p.tilesN = (N + SM - 1) / SM;
+        const int64_t t128 = (int64_t)p.tilesM * p.tilesN * batch;
+        int s128 = 1;
+        if (t128 < 256 && K32 >= 8 * BK) s128 = (int)std::min<int64_t>((256 + t128 - 1) / t128, K32 / (4 * BK));
+        p.kchunk = K32;
+        if (s128 > 1) {
+            p.kchunk = ((K32 + s128 - 1) / s128 + BK - 1) / BK * BK;
+            s128 = (K32 + p.kchunk - 1) / p.kchunk;
+        }
+        p.splits = s128;
+        dim3 grid(p.tilesM * p.tilesN, 1, batch * s128), block(256);
+        if (s128 > 1) {
+            void* w;
+            CT_CHECK(ws_get(h, WS_SPLITK, (size_t)batch * s128 * M * N * 4, &w));
+            p.part = (float*)w;
+            hipLaunchKernelGGL(gemm_bf16x6_nt_kernel<float>, grid, block, 0, h->stream, q);
+            if (tC == CT_F64)
+                splitk_reduce<double>(h->stream, p.part, (double*)C, M, N, s128, batch, ldc, bsC, alpha, beta, active);
+            else
+                splitk_reduce<float>(h->stream, p.part, (float*)C, M, N, s128, batch, ldc, bsC, alpha, beta, active);
+        } else if (tC == CT_F64) {
+            hipLaunchKernelGGL(gemm_bf16x6_nt_kernel<double>, grid, block, 0, h->stream, q);
+        } else {
+            hipLaunchKernelGGL(gemm_bf16x6_nt_kernel<float>, grid, block, 0, h->stream, q);
+        }
+        CT_HIP(hipGetLastError());
+        return 0;
+    }
     const int per_xcd = (p.tilesM * p.tilesN) / 8;
     p.GM = per_xcd >= p.tilesN ? per_xcd / p.tilesN : 1;
     if (p.GM < 1) p.GM = 1;

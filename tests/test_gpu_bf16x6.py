@@ -82,9 +82,13 @@ def test_split_bf16x6_bit_exact(ctx, rows, K, s_row_is_1, dt):
 
 
 @pytest.mark.parametrize("M,N,K,dt", [
+    # 256x128 tiles
     (4096, 1024, 8192, torch.float64),  # the last matvec GEMM at D = 1024, w = 8: 128 tiles, split-K = 2
-    (1000, 1100, 3000, torch.float32),  # ragged M / N / K, split-K over a padded K
-    (64, 96, 4096, torch.float64),      # one tile: split-K = 8
+    (1000, 1100, 3000, torch.float32),  # ragged M / N / K, split-K = 4 over a padded K
+    (2048, 2048, 256, torch.float64),   # 128 tiles, single K range
+    # 128x128 tiles (fewer than 128 tiles of 256x128 after splitting K)
+    (64, 96, 4096, torch.float64),      # one tile: split-K = 32
+    (200, 300, 1000, torch.float32),    # ragged, split-K = 8 over a padded K
     (300, 190, 77, torch.float64),      # ragged, single K range
     (513, 129, 32, torch.float32),      # one K step
 ])
