@@ -400,7 +400,7 @@ constexpr int X6_BL = X6_AL + X6M * 64;     // B l plane: 128 x 64 B
 constexpr int X6_STAGE = X6_BL + X6N * 64;  // 72 KiB
 
 struct X6Args {
-    NtArgs nt;  // split rows, output and tiling (tilesM, tilesN count 256 x 128 tiles)
+    NtArgs nt;  // split rows, output and tiling (tilesM, tilesN count 256 x 128 or 128 x 128 tiles)
     const uint16_t *Al, *Bl;
     int64_t ldal, ldbl, bsAl, bsBl;
 };
