@@ -323,7 +323,11 @@ bool aligned16(const void* ptr, int64_t s1, int64_t s2) {
 }
 
 // Split K when the tiles alone would leave most CUs idle (small bond
-// dimensions): at least 8 K-tiles per split, aiming at >= 256 workgroups.
+// dimensions): at least 4 K-tiles per split, aiming at >= 256 workgroups.
+// A short K loop is bound by the load latency of each step (a lone
+// workgroup per CU hides none of it): on the [328 x 410] . [410 x 82]
+// products of a D = 82, w = 5 matvec, 3 splits of 9 K steps took 15.7 us
+// and the L = 64 XXZ sweep's Lanczos 0.126 s; 6 splits of 5 steps 0.108 s.
 // With long K, also when they give fewer than two workgroups per CU: a
 // lone workgroup has nothing to run while it waits at its per-step
 // barrier (the [4096 x 8192] . [8192 x 1024] product of the D = 1024
@@ -331,7 +335,7 @@ bool aligned16(const void* ptr, int64_t s1, int64_t s2) {
 void plan_splits(const GemmDesc& g, int BT, int BKT, int* splits_out, int* kchunk_out) {
     const int64_t tiles = (int64_t)((g.M + BT - 1) / BT) * ((g.N + BT - 1) / BT) * g.batch;
     int splits = 1;
-    if (tiles < 128 && g.K >= 16 * BK) splits = (int)std::min<int64_t>((256 + tiles - 1) / tiles, g.K / (8 * BK));
+    if (tiles < 128 && g.K >= 8 * BK) splits = (int)std::min<int64_t>((256 + tiles - 1) / tiles, g.K / (4 * BK));
     else if (tiles < 512 && g.K >= 128 * BK) splits = (int)std::min<int64_t>((512 + tiles - 1) / tiles, g.K / (64 * BK));
     int kchunk = g.K;
     if (splits > 1) {
