@@ -79,34 +79,24 @@ __global__ __launch_bounds__(256) void lz_dot_partial(const double* __restrict__
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-// what: 0 -> scal[idx] = sum; 1 -> alpha[idx] = sum;
-//       2 -> beta step idx: b = sqrt(sum); breakdown test; beta[idx] = b
-__global__ __launch_bounds__(256) void lz_reduce(const double* __restrict__ part, LzState* s, int what, int idx,
-                                                 double thr, int gated, double floor_rel = 0.0) {
-    if (gated && !s->active) return;
+// Sum of the NB partial sums, in the same order in every block of every
+// kernel, so that every block gets the same bits.
+__device__ __forceinline__ double lz_sum_partials(const double* __restrict__ part) {
     double acc = 0.0;
     for (int i = threadIdx.x; i < NB; i += blockDim.x) acc += part[i];
-    acc = block_sum256(acc);
+    return block_sum256(acc);
+}
+
+// what: 0 -> scal[idx] = sum; 1 -> alpha[idx] = sum
+__global__ __launch_bounds__(256) void lz_reduce(const double* __restrict__ part, LzState* s, int what, int idx,
+                                                 int gated) {
+    if (gated && !s->active) return;
+    const double acc = lz_sum_partials(part);
     if (threadIdx.x != 0) return;
-    if (what == 0) {
+    if (what == 0)
         s->scal[idx] = acc;
-    } else if (what == 1) {
+    else
         s->alpha[idx] = acc;
-    } else {
-        const double b = sqrt(acc);
-        double t = thr;
-        if (floor_rel > 0.0) {
-            double scale = 0.0;
-            for (int i = 0; i <= idx; ++i) scale = fmax(scale, fabs(s->alpha[i]));
-            t = fmax(t, floor_rel * scale);
-        }
-        if (b < t) {
-            s->numiter = idx + 1;
-            s->active = 0;
-        } else {
-            s->beta[idx] = b;
-        }
-    }
 }
 
 // v0 = theta / |theta|; zero start vector -> err (dmrg.mojo:605-613)
@@ -122,12 +112,17 @@ __global__ __launch_bounds__(256) void lz_start(const double* __restrict__ theta
         v0[i] = theta[i] * inv;
 }
 
-// w -= alpha_j v_j + beta_{j-1} v_{j-1}; partial sums of w.w (dmrg.mojo:656-669)
-__global__ __launch_bounds__(256) void lz_update(double* __restrict__ w, const double* __restrict__ vj,
-                                                 const double* __restrict__ vjm1, int64_t n, int j, const LzState* s,
-                                                 double* __restrict__ part) {
+// alpha_j = v_j.w from the partial sums of lz_dot_partial (block 0 records
+// it), then w -= alpha_j v_j + beta_{j-1} v_{j-1} and partial sums of w.w
+// (dmrg.mojo:649-669).  Every block forms alpha_j itself from the partials,
+// which saves a reduction launch per Krylov step.
+__global__ __launch_bounds__(256) void lz_update_alpha(double* __restrict__ w, const double* __restrict__ vj,
+                                                       const double* __restrict__ vjm1, int64_t n, int j, LzState* s,
+                                                       const double* __restrict__ part_in,
+                                                       double* __restrict__ part_out) {
     if (!s->active) return;
-    const double a = s->alpha[j];
+    const double a = lz_sum_partials(part_in);
+    if (blockIdx.x == 0 && threadIdx.x == 0) s->alpha[j] = a;
     const double b = j > 0 ? s->beta[j - 1] : 0.0;
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -137,14 +132,36 @@ __global__ __launch_bounds__(256) void lz_update(double* __restrict__ w, const d
         acc += x * x;
     }
     acc = block_sum256(acc);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part_out[blockIdx.x] = acc;
 }
 
-// v_{j+1} = w / beta_j (dmrg.mojo:677-681)
-__global__ __launch_bounds__(256) void lz_next(const double* __restrict__ w, double* __restrict__ vn, int64_t n, int j,
-                                               const LzState* s) {
-    if (!s->active) return;
-    const double inv = 1.0 / s->beta[j];
+// beta_j = |w| from the partial sums of lz_update_alpha, the breakdown test
+// (beta_j below thr, or below floor_rel * max|alpha_i| for the mixed-precision
+// forms: the Krylov space ends, numiter = j + 1), else v_{j+1} = w / beta_j
+// (dmrg.mojo:671-681).  Every block forms beta_j and takes the same decision;
+// block 0 records it.  The state's active flag is read and written with
+// atomics: block 0 may clear it while other blocks of this launch start.
+__global__ __launch_bounds__(256) void lz_next_beta(const double* __restrict__ w, double* __restrict__ vn, int64_t n,
+                                                    int j, LzState* s, const double* __restrict__ part, double thr,
+                                                    double floor_rel) {
+    if (__hip_atomic_load(&s->active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    const double b = sqrt(lz_sum_partials(part));
+    double t = thr;
+    if (floor_rel > 0.0) {
+        double scale = 0.0;
+        for (int i = 0; i <= j; ++i) scale = fmax(scale, fabs(s->alpha[i]));
+        t = fmax(t, floor_rel * scale);
+    }
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (b < t) {
+        if (lead) {
+            s->numiter = j + 1;
+            __hip_atomic_store(&s->active, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    if (lead) s->beta[j] = b;
+    const double inv = 1.0 / b;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         vn[i] = w[i] * inv;
 }
@@ -579,13 +596,14 @@ int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double*
     void *kb, *wb, *pb, *sb;
     CT_CHECK(ws_get(h, WS_KRYLOV, (size_t)dim * max_iter * 8, &kb));
     CT_CHECK(ws_get(h, WS_W, (size_t)dim * 3 * 8, &wb));
-    CT_CHECK(ws_get(h, WS_PARTIAL, NB * 8, &pb));
+    CT_CHECK(ws_get(h, WS_PARTIAL, 2 * NB * 8, &pb));
     CT_CHECK(ws_get(h, WS_LZ, sizeof(LzState), &sb));
     double* K = (double*)kb;
     double* w = (double*)wb;
     double* w2 = w + dim;
     double* hv0 = w + 2 * dim;  // FP64 H v0, kept for the refinement (exact_start)
     double* part = (double*)pb;
+    double* part2 = part + NB;  // w.w partials of a Krylov step (part holds its v_j.w partials)
     LzState* s = (LzState*)sb;
     const int* act = &s->active;
     const double thr = 100.0 * (double)dim * 2.2204460492503131e-16;
@@ -609,32 +627,31 @@ int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double*
     hipLaunchKernelGGL(lz_init_kernel, dim3(1), dim3(1), 0, st, s, max_iter, exact_start ? 1 : 0);
     hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)d_theta, (const double*)nullptr, dim,
                        part, (const LzState*)s, 0);
-    hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 0, thr, 0);
+    hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 0, 0);
     hipLaunchKernelGGL(lz_start, dim3(G), dim3(256), 0, st, (const double*)d_theta, K, dim, s);
     for (int j = 0; j < max_iter - 1; ++j) {
         double* vj = K + (int64_t)j * dim;
         CT_CHECK(krylov_matvec(j, vj));
         hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)vj, (const double*)w, dim, part,
                            (const LzState*)s, 1);
-        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, j, thr, 1);
-        hipLaunchKernelGGL(lz_update, dim3(NB), dim3(256), 0, st, w, (const double*)vj,
-                           (const double*)(j > 0 ? vj - dim : vj), dim, j, (const LzState*)s, part);
-        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 2, j, thr, 1, floor_rel);
-        hipLaunchKernelGGL(lz_next, dim3(G), dim3(256), 0, st, (const double*)w, vj + dim, dim, j, (const LzState*)s);
+        hipLaunchKernelGGL(lz_update_alpha, dim3(NB), dim3(256), 0, st, w, (const double*)vj,
+                           (const double*)(j > 0 ? vj - dim : vj), dim, j, s, (const double*)part, part2);
+        hipLaunchKernelGGL(lz_next_beta, dim3(G), dim3(256), 0, st, (const double*)w, vj + dim, dim, j, s,
+                           (const double*)part2, thr, floor_rel);
     }
     {   // final alpha-only step when no breakdown happened (dmrg.mojo:691-703)
         double* vj = K + (int64_t)(max_iter - 1) * dim;
         CT_CHECK(krylov_matvec(max_iter - 1, vj));
         hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)vj, (const double*)w, dim, part,
                            (const LzState*)s, 1);
-        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, max_iter - 1, thr, 1);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 1, max_iter - 1, 1);
     }
     if (max_rot < 0)
         hipLaunchKernelGGL(lz_tridiag_exact, dim3(1), dim3(64), 0, st, s);
     else
         hipLaunchKernelGGL(lz_jacobi, dim3(1), dim3(64), 0, st, s, tol, max_rot);
     hipLaunchKernelGGL(lz_ritz, dim3(NB), dim3(256), 0, st, (const double*)K, dim, d_theta, (const LzState*)s, part);
-    hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 2, thr, 0);
+    hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 2, 0);
     hipLaunchKernelGGL(lz_normalize, dim3(G), dim3(256), 0, st, d_theta, dim, (const LzState*)s, 2);
     CT_HIP(hipGetLastError());
 
@@ -647,17 +664,17 @@ int lanczos_enqueue(Handle* h, const HeffEnv& env, const HeffEnv* env64, double*
         CT_CHECK(heff_apply(h, *env64, CT_F64, d_theta, w, exact_start ? &s->refmv : nullptr));
         hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)d_theta, (const double*)w, dim,
                            part, (const LzState*)s, 0);
-        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 3, thr, 0);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 3, 0);
         hipLaunchKernelGGL(lz_residual, dim3(NB), dim3(256), 0, st, w, (const double*)d_theta, dim, (const LzState*)s,
                            part);
-        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 4, thr, 0);
+        hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 4, 0);
         if (refine >= 2) {
             hipLaunchKernelGGL(lz_rhat, dim3(G), dim3(256), 0, st, (const double*)w, w2, dim, (const LzState*)s);
             // h_rr = <rhat|H rhat> at the Krylov precision (second-order effect on the energy)
             CT_CHECK(heff_apply(h, env, CT_F64, w2, w, nullptr));
             hipLaunchKernelGGL(lz_dot_partial, dim3(NB), dim3(256), 0, st, (const double*)w2, (const double*)w, dim,
                                part, (const LzState*)s, 0);
-            hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 5, thr, 0);
+            hipLaunchKernelGGL(lz_reduce, dim3(1), dim3(256), 0, st, (const double*)part, s, 0, 5, 0);
             hipLaunchKernelGGL(lz_correct, dim3(G), dim3(256), 0, st, d_theta, (const double*)w2, dim, s);
         }
         if (exact_start) {
