@@ -5,6 +5,8 @@ generator matches the oracle bit for bit."""
 import ctypes
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -63,3 +65,47 @@ def test_gpu_entry_points_fail_loudly_without_gpu():
         pytest.skip("a GPU is present")
     with pytest.raises(RuntimeError):
         ct.DeviceContext(0)
+
+
+C_CLIENT = r"""
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include "chemtensor_hip.h"
+
+int main(void) {
+    /* the header is plain C: enums, handles, pointers and sizes only */
+    const int precs[] = {CT_PREC_FP64, CT_PREC_FP32, CT_PREC_BF16X3, CT_PREC_BF16X6};
+    int64_t dims[4];
+    double sites[64];
+    int64_t offs[4];
+    if (strstr(ct_version(), "gfx950") == NULL) return 1;
+    if (ct_synchronize(NULL) != -1) return 2;
+    if (ct_random_mps_c_compatible(3, 2, 0, 1, dims, sites, offs) != -3) return 3;
+    if (strstr(ct_last_error(), "chi_max") == NULL) return 4;
+    printf("%d %d %d %d\n", precs[0], precs[1], precs[2], precs[3]);
+    return 0;
+}
+"""
+
+
+def test_header_is_a_c_abi_for_a_c_host(tmp_path):
+    """A C99 program that includes include/chemtensor_hip.h compiles with
+    -Wall -Werror -pedantic (no C++ or torch types in the boundary), links
+    against the library and gets its LAPACK-style argument statuses without
+    a device: what a non-Python host (the reference's Mojo FFI, cgo, JNI)
+    binds to."""
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    _lib.load()  # fails if the library has not been built
+    src = tmp_path / "client.c"
+    src.write_text(C_CLIENT)
+    exe = tmp_path / "client"
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-pedantic", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-o", str(exe), "-L", libdir, "-l:" + os.path.basename(_lib.LIB_PATH),
+                    "-Wl,-rpath," + libdir], check=True, capture_output=True, text=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, (out.returncode, out.stderr)
+    assert out.stdout.split() == ["0", "1", "2", "3"]
