@@ -203,7 +203,7 @@ def bench_env_theta(ctx, steps, warmup, D=1024, ws=(5, 8)):
     return out
 
 
-def bench_matvec_grid(ctx, steps, warmup, Ds=(256, 512, 1024), ws=(5, 8)):
+def bench_matvec_grid(ctx, steps, warmup, Ds=(256, 512, 1024, 2048), ws=(5, 8)):
     """Prepared-operator matvec rate (TFLOP/s, mean over `steps` calls timed with HIP events) over D and w: the
     microbenchmark grid (XXZ bulk MPO for w=5, J1-J2 bulk MPO for w=8)."""
     import torch
