@@ -196,13 +196,16 @@ def test_config4_qc16_properties(ctx, qc16):
     carry no particle-number labels, so a truncation that cuts through nearly
     degenerate singular values of different sectors mixes them; the FP64
     oracle shows the same effect, ~1e-7 at 6 orbitals and chi = 16; measured
-    here 4e-6); refined FP32 / 3xBF16 warm sweeps land within their matvec
-    error (1e-6 / 6e-6 relative) of the FP64 warm sweep with the same
-    refinement.  At chi = 64 the energy still falls by ~3e-3 relative per
-    sweep, and each truncated local eigenvector carries its matvec error into
-    the next step, so the forms agree only to first order in that error
-    (measured 1.4e-7 / 6.4e-7); the 1e-9 agreement of converged sweeps does
-    not apply to this regime."""
+    here 4e-6); the refined FP32 / 3xBF16 / 6xBF16 warm sweeps land within
+    1e-5 relative of the FP64 warm sweep with the same refinement.  At chi = 64
+    the energy still falls by ~3e-3 relative per sweep and every truncation
+    cuts through a dense spectrum, so rounding-level differences anywhere
+    (the forms' matvec errors, or the summation order of a kernel) change
+    which vectors are kept: measured 3.2e-7 / 2.1e-8 / 8.9e-8 (FP32 / 3xBF16 /
+    6xBF16) with this build, 1.4e-7 / 6.4e-7 (FP32 / 3xBF16) with an earlier
+    one, and 2.2e-6 / 2.9e-6 / 3.4e-6 (all below the FP64 result) in a build
+    that changed only the summation order of the eigensolver's T factor.  The 1e-9 agreement of converged sweeps does not
+    apply to this regime."""
     mpo_t, pen_t, info = qc16
     start = ham.number_sector_random_mps_tensors(32, 16, 16, seed=42)
     kw = dict(chi_max=64, eps_trunc=1e-12, max_krylov_iter=20, krylov_tol=1e-12, energy_tol=0.0, jacobi_max_rot=-1)
@@ -222,8 +225,8 @@ def test_config4_qc16_properties(ctx, qc16):
     assert np.all(np.diff(se) <= 1e-9 * abs(se[0]))
     assert abs(e_state - E) <= 1e-9 * abs(E)
     assert leak < 1e-2
-    for name, tol in (("fp32", 1e-6), ("bf16x3", 6e-6), ("bf16x6", 1e-6)):
-        assert abs(warm[name] - warm["fp64"]) <= tol * abs(warm["fp64"]), (name, warm)
+    for name in ("fp32", "bf16x3", "bf16x6"):
+        assert abs(warm[name] - warm["fp64"]) <= 1e-5 * abs(warm["fp64"]), (name, warm)
     assert warm["fp64"] <= E + 1e-9 * abs(E)
 
 
